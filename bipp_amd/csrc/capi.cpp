@@ -1,0 +1,991 @@
+// Host driver and C-ABI of libbipp_amd: contexts, the batched eigensolve and the fused
+// synthesis pipeline (stage A -> level weights -> image accumulation) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "bipp_hip.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace bippamd {
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+BippError guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return BIPP_SUCCESS;
+  } catch (const BippException& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return BIPP_GENERIC_ERROR;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BIPP_GENERIC_ERROR;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return BIPP_GENERIC_ERROR;
+  }
+}
+
+void require(bool cond, BippError code, const char* msg) {
+  if (!cond) fail(code, msg);
+}
+
+// Growable device allocation.  Reallocation frees the old block; hipFree synchronises the
+// device, so work still in flight on it has finished first.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void* get(size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > cap) {
+      if (p) BIPP_HIP(hipFree(p));
+      p = nullptr;
+      cap = 0;
+      BIPP_HIP(hipMalloc(&p, bytes));
+      cap = bytes;
+    }
+    return p;
+  }
+  template <typename T>
+  T* as(size_t n) {
+    return static_cast<T*>(get(n * sizeof(T)));
+  }
+};
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void* get(size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > cap) {
+      if (p) BIPP_HIP(hipHostFree(p));
+      p = nullptr;
+      cap = 0;
+      BIPP_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+      cap = bytes;
+    }
+    return p;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------------------
+struct ContextImpl {
+  BippProcessingUnit pu = BIPP_PU_GPU;
+  int device = 0;
+  int n_cu = 1;
+  hipStream_t stream = nullptr;
+
+  explicit ContextImpl(BippProcessingUnit p) {
+    require(p != BIPP_PU_CPU, BIPP_INVALID_PARAMETER_ERROR,
+            "this library computes on the GPU only (processing unit CPU requested)");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      fail(BIPP_GPU_SUPPORT_ERROR, "no GPU device available");
+    BIPP_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    BIPP_HIP(hipGetDeviceProperties(&prop, device));
+    n_cu = prop.multiProcessorCount;
+    BIPP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pu = BIPP_PU_GPU;
+  }
+  ~ContextImpl() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void sync() { BIPP_HIP(hipStreamSynchronize(stream)); }
+};
+
+// ---------------------------------------------------------------------------------------
+// Stage A on a batch: flag -> [Gram/W congruence] -> generalized eigensolve.
+// Results stay on the device: d (R, stride nmax), z (reduced eigenvectors, ld nmax) and,
+// for a general W, v_full = W_I Z (nAntenna x nmax).
+// ---------------------------------------------------------------------------------------
+template <typename R>
+struct EigRunner {
+  DevBuf n_red, idx, n_vis, info, counter, order, d, z, ws_a, ws_l, ws_z;
+  DevBuf sig, w_i, tmp, g, v_full;
+  int n_samples = 0, n_ant = 0, nmax = 0;
+  bool general_w = false;
+  std::vector<int> h_info;
+
+  void run(ContextImpl& ctx, int nS, int nAnt, int nBeam, const double* wl, const void* s,
+           bool s_double, long long lds, long long stride_s, const void* w, bool w_double,
+           long long ldw, long long stride_w, const double* xyz, long long ldxyz,
+           long long stride_xyz, bool standard, double prefactor) {
+    require(nBeam >= 1 && nBeam <= 512, BIPP_INVALID_PARAMETER_ERROR,
+            "number of beams must be in [1, 512]");
+    require(nAnt >= 1, BIPP_INVALID_PARAMETER_ERROR, "number of antennas must be positive");
+    require(w != nullptr || nAnt == nBeam, BIPP_INVALID_PARAMETER_ERROR,
+            "identity beamforming needs nAntenna == nBeam");
+    require(lds >= nBeam, BIPP_INVALID_PARAMETER_ERROR, "lds < nBeam");
+    n_samples = nS;
+    n_ant = nAnt;
+    nmax = nBeam;
+    general_w = w != nullptr;
+    hipStream_t st = ctx.stream;
+    const long long nn = (long long)nmax * nmax;
+
+    FlagParams fp;
+    fp.n_beam = nBeam;
+    fp.mode = standard ? 1 : 0;
+    fp.s = s;
+    fp.lds = lds;
+    fp.stride_s = stride_s;
+    fp.n_red = n_red.as<int>(nS);
+    fp.idx = idx.as<int>((size_t)nS * nmax);
+    fp.idx_stride = nmax;
+    fp.n_vis = n_vis.as<unsigned long long>(nS);
+    launch_flag(fp, nS, s_double, st);
+
+    const cplx<R>* g_in = nullptr;
+    if (general_w && !standard) {
+      // G_r = W_I^H Sigma W_I (src/host/gram_matrix.cpp:58-61)
+      cplx<R>* sg = sig.as<cplx<R>>((size_t)nS * nAnt * nAnt);
+      launch_sinc_matrix<R>(nS, nAnt, xyz, ldxyz, stride_xyz, wl, prefactor, sg, nAnt,
+                            (long long)nAnt * nAnt, st);
+      cplx<R>* wi = w_i.as<cplx<R>>((size_t)nS * nAnt * nmax);
+      if (w_double)
+        launch_gather_w<R, double>(nS, nAnt, nBeam, w, ldw, stride_w, fp.n_red, fp.idx, nmax, wi,
+                                   nAnt, (long long)nAnt * nmax, st);
+      else
+        launch_gather_w<R, float>(nS, nAnt, nBeam, w, ldw, stride_w, fp.n_red, fp.idx, nmax, wi,
+                                  nAnt, (long long)nAnt * nmax, st);
+      cplx<R>* t = tmp.as<cplx<R>>((size_t)nS * nAnt * nmax);
+      GemmBatch<R> g1{};
+      g1.n_batch = nS;
+      g1.m = nullptr;
+      g1.m0 = nAnt;
+      g1.n = fp.n_red;
+      g1.k = nullptr;
+      g1.k0 = nAnt;
+      g1.op_a = 0;
+      g1.op_b = 0;
+      g1.alpha = mk<R>(1, 0);
+      g1.beta = mk<R>(0, 0);
+      g1.a = sg;
+      g1.lda = nAnt;
+      g1.stride_a = (long long)nAnt * nAnt;
+      g1.b = wi;
+      g1.ldb = nAnt;
+      g1.stride_b = (long long)nAnt * nmax;
+      g1.c = t;
+      g1.ldc = nAnt;
+      g1.stride_c = (long long)nAnt * nmax;
+      launch_gemm_batch<R>(g1, nAnt, nmax, st);
+      cplx<R>* gr = g.as<cplx<R>>((size_t)nS * nn);
+      GemmBatch<R> g2 = g1;
+      g2.m = fp.n_red;
+      g2.n = fp.n_red;
+      g2.k = nullptr;
+      g2.k0 = nAnt;
+      g2.op_a = 1;
+      g2.a = wi;
+      g2.lda = nAnt;
+      g2.stride_a = (long long)nAnt * nmax;
+      g2.b = t;
+      g2.ldb = nAnt;
+      g2.stride_b = (long long)nAnt * nmax;
+      g2.c = gr;
+      g2.ldc = nmax;
+      g2.stride_c = nn;
+      launch_gemm_batch<R>(g2, nmax, nmax, st);
+      g_in = gr;
+    }
+
+    // persistent launch, largest problems first
+    std::vector<int> h_nred(nS);
+    BIPP_HIP(hipMemcpyAsync(h_nred.data(), fp.n_red, sizeof(int) * nS, hipMemcpyDeviceToHost, st));
+    BIPP_HIP(hipStreamSynchronize(st));
+    std::vector<int> h_order(nS);
+    std::iota(h_order.begin(), h_order.end(), 0);
+    std::stable_sort(h_order.begin(), h_order.end(),
+                     [&](int a, int b) { return h_nred[a] > h_nred[b]; });
+    int* d_order = order.as<int>(nS);
+    BIPP_HIP(hipMemcpyAsync(d_order, h_order.data(), sizeof(int) * nS, hipMemcpyHostToDevice, st));
+
+    const int per_cu = eig_blocks_per_cu<R>(nmax, s_double);
+    const int n_slots = std::max(1, std::min(nS, per_cu * ctx.n_cu));
+    EigParams<R> ep;
+    ep.n_jobs = nS;
+    ep.nmax = nmax;
+    ep.order = d_order;
+    ep.counter = counter.as<unsigned>(1);
+    launch_fill_u32(ep.counter, 0u, 1, st);
+    ep.n_red = fp.n_red;
+    ep.idx = fp.idx;
+    ep.idx_stride = nmax;
+    ep.s = s;
+    ep.lds = lds;
+    ep.stride_s = stride_s;
+    ep.xyz = xyz;
+    ep.ldxyz = ldxyz;
+    ep.stride_xyz = stride_xyz;
+    ep.wl = wl;
+    ep.g_in = g_in;
+    ep.stride_g = nn;
+    ep.ld_g = nmax;
+    ep.gram_prefactor = prefactor;
+    ep.standard = standard ? 1 : 0;
+    ep.d_out = d.as<R>((size_t)nS * nmax);
+    ep.stride_d = nmax;
+    ep.z_out = z.as<cplx<R>>((size_t)nS * nn);
+    ep.stride_z = nn;
+    ep.ldz = nmax;
+    ep.ws_ld = nmax;
+    ep.ws_stride = nn;
+    ep.ws_a = ws_a.as<cplx<R>>((size_t)n_slots * nn);
+    ep.ws_l = ws_l.as<cplx<R>>((size_t)n_slots * nn);
+    ep.ws_z = ws_z.as<R>((size_t)n_slots * nn);
+    ep.info = info.as<int>(nS);
+    launch_eig<R>(ep, n_slots, s_double, st);
+
+    if (general_w) {
+      // V = W_I Z (src/eigensolver.cpp:214-216)
+      cplx<R>* vf = v_full.as<cplx<R>>((size_t)nS * nAnt * nmax);
+      BIPP_HIP(hipMemsetAsync(vf, 0, sizeof(cplx<R>) * (size_t)nS * nAnt * nmax, st));
+      GemmBatch<R> g3{};
+      g3.n_batch = nS;
+      g3.m = nullptr;
+      g3.m0 = nAnt;
+      g3.n = fp.n_red;
+      g3.k = fp.n_red;
+      g3.op_a = 0;
+      g3.op_b = 0;
+      g3.alpha = mk<R>(1, 0);
+      g3.beta = mk<R>(0, 0);
+      g3.a = static_cast<const cplx<R>*>(w_i.p);
+      g3.lda = nAnt;
+      g3.stride_a = (long long)nAnt * nmax;
+      g3.b = ep.z_out;
+      g3.ldb = nmax;
+      g3.stride_b = nn;
+      g3.c = vf;
+      g3.ldc = nAnt;
+      g3.stride_c = (long long)nAnt * nmax;
+      launch_gemm_batch<R>(g3, nAnt, nmax, st);
+    }
+  }
+
+  // Throws EigensolverError if any solve failed (LAPACK info != 0 analogue).
+  void check(ContextImpl& ctx) {
+    h_info.resize(n_samples);
+    BIPP_HIP(hipMemcpyAsync(h_info.data(), info.p, sizeof(int) * n_samples, hipMemcpyDeviceToHost,
+                            ctx.stream));
+    BIPP_HIP(hipStreamSynchronize(ctx.stream));
+    for (int k = 0; k < n_samples; ++k)
+      if (h_info[k] != 0)
+        fail(BIPP_EIGENSOLVER_ERROR, "generalized eigensolve failed for sample " +
+                                         std::to_string(k) + " (info " +
+                                         std::to_string(h_info[k]) + ")");
+  }
+
+  const int* d_n_red() const { return static_cast<const int*>(n_red.p); }
+  const int* d_idx() const { return static_cast<const int*>(idx.p); }
+  const unsigned long long* d_n_vis() const {
+    return static_cast<const unsigned long long*>(n_vis.p);
+  }
+  const R* d_d() const { return static_cast<const R*>(d.p); }
+  const cplx<R>* d_z() const { return static_cast<const cplx<R>*>(z.p); }
+  const cplx<R>* d_vfull() const { return static_cast<const cplx<R>*>(v_full.p); }
+};
+
+// Writes d, V (W = I scatter or W_I Z), scale and kept counts in the caller's layout.
+template <typename R>
+void export_eig(ContextImpl& ctx, EigRunner<R>& er, int nS, int nAnt, int nBeam, bool out_double,
+                void* d, long long stride_d, void* v, long long ldv, long long stride_v,
+                int32_t* n_red_out, void* scale) {
+  hipStream_t st = ctx.stream;
+  launch_cast_d<R>(nS, nBeam, er.d_d(), nBeam, d, stride_d, out_double, er.d_n_vis(), scale, st);
+  if (v) {
+    if (!er.general_w) {
+      UnbeamParams<R> up;
+      up.n_ant = nAnt;
+      up.n_red = er.d_n_red();
+      up.idx = er.d_idx();
+      up.idx_stride = nBeam;
+      up.z = er.d_z();
+      up.stride_z = (long long)nBeam * nBeam;
+      up.ldz = nBeam;
+      up.v = v;
+      up.stride_v = stride_v;
+      up.ldv = ldv;
+      launch_unbeam_identity<R>(up, nS, nBeam, out_double, st);
+    } else if (out_double) {
+      launch_copy_cast<R, double>(nS, nAnt, nBeam, er.d_vfull(), nAnt, (long long)nAnt * nBeam, v,
+                                  ldv, stride_v, st);
+    } else {
+      launch_copy_cast<R, float>(nS, nAnt, nBeam, er.d_vfull(), nAnt, (long long)nAnt * nBeam, v,
+                                 ldv, stride_v, st);
+    }
+  }
+  if (n_red_out)
+    BIPP_HIP(hipMemcpyAsync(n_red_out, er.d_n_red(), sizeof(int) * nS, hipMemcpyDeviceToDevice, st));
+}
+
+}  // namespace bippamd
+
+using namespace bippamd;
+
+struct BippContextImpl {
+  std::unique_ptr<ContextImpl> impl;
+  EigRunner<float> eig32;
+  EigRunner<double> eig64;
+};
+
+// ---------------------------------------------------------------------------------------
+// Fused synthesis pipeline
+// ---------------------------------------------------------------------------------------
+struct BippSynthesisImpl {
+  BippContextImpl* ctx = nullptr;
+  BippCompute compute = BIPP_COMPUTE_MIXED;
+  int imager = 0;
+  int n_ant = 0, n_beam = 0, n_levels = 0, n_pix = 0;
+  BippFilter filter = BIPP_FILTER_LSQ;
+  bool batched = true;
+  size_t n_collected = 0;
+  DevBuf intervals, lmn64, lmn32, img;
+  // steering buffers
+  DevBuf n_eig, n_ant_used, pos, afrag, wts;
+  // direct-sum buffers
+  DevBuf wdense, vfull, vv, vv32, uvw;
+  // staging for host collects
+  DevBuf st_s, st_w, st_xyz, st_wl;
+  std::vector<double> last_d;
+  std::vector<int32_t> last_nred;
+  int last_count = 0;
+
+  bool img_double() const { return compute == BIPP_COMPUTE_FP64; }
+  int steer_mode() const {
+    return compute == BIPP_COMPUTE_FP64 ? STEER_FP64
+                                        : (compute == BIPP_COMPUTE_FP32 ? STEER_FP32 : STEER_BF16X3);
+  }
+
+  template <typename R>
+  void collect_impl(EigRunner<R>& er, int nS, const double* wl, bool s_double, const void* s,
+                    long long lds, long long stride_s, const void* w, long long ldw,
+                    long long stride_w, const double* xyz, long long ldxyz, long long stride_xyz) {
+    ContextImpl& c = *ctx->impl;
+    hipStream_t st = c.stream;
+    er.run(c, nS, n_ant, n_beam, wl, s, s_double, lds, stride_s, w, s_double, ldw, stride_w, xyz,
+           ldxyz, stride_xyz, false, 1.0);
+    const double* d_int = static_cast<const double*>(intervals.p);
+    if (imager == 0) {
+      const int mode = steer_mode();
+      const int apad = ((n_ant + 15) / 16) * 16;
+      const int kchunks = apad / 16;
+      const int rtiles = (2 * n_beam + 15) / 16;
+      const int epad = rtiles * 8;
+      const size_t fe = steer_frag_elems_per_sample(mode, rtiles, kchunks);
+      const size_t esz = mode == STEER_BF16X3 ? 2 : (mode == STEER_FP32 ? 4 : 8);
+      SteerPrepParams<R> pp;
+      pp.n_samples = nS;
+      pp.n_levels = n_levels;
+      pp.filter = filter;
+      pp.intervals = d_int;
+      pp.n_red = er.d_n_red();
+      pp.idx = er.general_w ? nullptr : er.d_idx();
+      pp.idx_stride = n_beam;
+      pp.d = er.d_d();
+      pp.stride_d = n_beam;
+      pp.n_cols = n_beam;
+      pp.n_vis = er.d_n_vis();
+      if (er.general_w) {
+        pp.z = er.d_vfull();
+        pp.ldz = n_ant;
+        pp.stride_z = (long long)n_ant * n_beam;
+      } else {
+        pp.z = er.d_z();
+        pp.ldz = n_beam;
+        pp.stride_z = (long long)n_beam * n_beam;
+      }
+      pp.xyz = xyz;
+      pp.ldxyz = ldxyz;
+      pp.stride_xyz = stride_xyz;
+      pp.wl = wl;
+      pp.n_eig = n_eig.as<int>(nS);
+      pp.n_ant = n_ant_used.as<int>(nS);
+      pp.pos_stride = 3LL * apad;
+      pp.pos = pos.as<double>((size_t)nS * pp.pos_stride);
+      pp.apad = apad;
+      pp.afrag_stride = static_cast<long long>(fe);
+      pp.afrag = afrag.get(nS * fe * esz);
+      pp.kchunks = kchunks;
+      pp.rtiles = rtiles;
+      pp.w_stride = (long long)n_levels * epad;
+      pp.wts = wts.get((size_t)nS * pp.w_stride * (mode == STEER_FP64 ? 8 : 4));
+      pp.epad = epad;
+      pp.mode = mode;
+      // rows of V_full are all antennas; n_red then counts eigenvectors only
+      pp.n_rows_const = er.general_w ? n_ant : 0;
+      launch_steer_prep<R>(pp, st);
+      SteerParams sp;
+      sp.n_samples = nS;
+      sp.n_levels = n_levels;
+      sp.n_pix = n_pix;
+      sp.n_eig = pp.n_eig;
+      sp.n_ant = pp.n_ant;
+      sp.pos = pp.pos;
+      sp.pos_stride = pp.pos_stride;
+      sp.apad = apad;
+      sp.afrag = pp.afrag;
+      sp.afrag_stride = pp.afrag_stride;
+      sp.kchunks = kchunks;
+      sp.rtiles = rtiles;
+      sp.wts = pp.wts;
+      sp.w_stride = pp.w_stride;
+      sp.epad = epad;
+      sp.lmn = static_cast<const double*>(lmn64.p);
+      sp.phase_fp32 = compute == BIPP_COMPUTE_FP32 ? 1 : 0;
+      sp.img = img.p;
+      sp.img_ld = n_pix;
+      launch_steer(sp, mode, st);
+    } else {
+      // virtual visibilities + direct NUFT sum, per sample and level
+      R* wd = wdense.as<R>((size_t)nS * n_levels * n_beam);
+      launch_level_weights<R>(nS, n_levels, filter, d_int, er.d_n_red(), er.d_d(), n_beam, n_beam,
+                              er.d_n_vis(), wd, (long long)n_levels * n_beam, st);
+      cplx<R>* vf;
+      if (er.general_w) {
+        vf = const_cast<cplx<R>*>(er.d_vfull());
+      } else {
+        vf = vfull.as<cplx<R>>((size_t)nS * n_ant * n_beam);
+        UnbeamParams<R> up;
+        up.n_ant = n_ant;
+        up.n_red = er.d_n_red();
+        up.idx = er.d_idx();
+        up.idx_stride = n_beam;
+        up.z = er.d_z();
+        up.stride_z = (long long)n_beam * n_beam;
+        up.ldz = n_beam;
+        up.v = vf;
+        up.stride_v = (long long)n_ant * n_beam;
+        up.ldv = n_ant;
+        launch_unbeam_identity<R>(up, nS, n_beam, sizeof(R) == 8, st);
+      }
+      const long long nn = (long long)n_ant * n_ant;
+      cplx<R>* vvb = vv.as<cplx<R>>((size_t)nn);
+      std::vector<double> h_wl(nS);
+      BIPP_HIP(hipMemcpyAsync(h_wl.data(), wl, sizeof(double) * nS, hipMemcpyDeviceToHost, st));
+      BIPP_HIP(hipStreamSynchronize(st));
+      const bool mixed = compute == BIPP_COMPUTE_MIXED;
+      for (int k = 0; k < nS; ++k) {
+        const double uvw_scale = 2.0 * 3.14159265358979323846 / h_wl[k];
+        const double* xk = xyz + (size_t)k * stride_xyz;
+        for (int lv = 0; lv < n_levels; ++lv) {
+          launch_virtual_vis<R>(1, n_ant, n_beam, vf + (size_t)k * n_ant * n_beam, n_ant, 0,
+                                wd + ((size_t)k * n_levels + lv) * n_beam, 0, vvb, 0, st);
+          if (mixed) {
+            double* u = uvw.as<double>(3 * (size_t)nn);
+            launch_baselines<double>(n_ant, xk, ldxyz, uvw_scale, u, u + nn, u + 2 * nn, st);
+            cplx<float>* v32 = vv32.as<cplx<float>>((size_t)nn);
+            launch_copy_cast<R, float>(1, (int)nn, 1, vvb, nn, 0, v32, nn, 0, st);
+            const double* l = static_cast<const double*>(lmn64.p);
+            launch_nuft_sum_mixed(1.0, (int)nn, v32, u, u + nn, u + 2 * nn, n_pix, l, l + n_pix,
+                                  l + 2 * (size_t)n_pix, static_cast<float*>(img.p) + (size_t)lv * n_pix,
+                                  st);
+          } else {
+            R* u = uvw.as<R>(3 * (size_t)nn);
+            launch_baselines<R>(n_ant, xk, ldxyz, uvw_scale, u, u + nn, u + 2 * nn, st);
+            const R* l = sizeof(R) == 8 ? static_cast<const R*>(lmn64.p) : static_cast<const R*>(lmn32.p);
+            launch_nuft_sum<R>(R(1), (int)nn, vvb, u, u + nn, u + 2 * nn, n_pix, l, l + n_pix,
+                               l + 2 * (size_t)n_pix, static_cast<R*>(img.p) + (size_t)lv * n_pix, st);
+          }
+        }
+      }
+    }
+    er.check(c);
+    // keep the eigenvalues of this call for inspection
+    last_count = nS;
+    last_d.resize((size_t)nS * n_beam);
+    last_nred.resize(nS);
+    std::vector<R> tmp((size_t)nS * n_beam);
+    BIPP_HIP(hipMemcpyAsync(tmp.data(), er.d_d(), sizeof(R) * tmp.size(), hipMemcpyDeviceToHost, st));
+    BIPP_HIP(hipMemcpyAsync(last_nred.data(), er.d_n_red(), sizeof(int) * nS, hipMemcpyDeviceToHost, st));
+    BIPP_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < tmp.size(); ++i) last_d[i] = static_cast<double>(tmp[i]);
+    n_collected += nS;
+  }
+
+  void collect(int nS, const double* wl, bool s_double, const void* s, long long lds,
+               long long stride_s, const void* w, long long ldw, long long stride_w,
+               const double* xyz, long long ldxyz, long long stride_xyz) {
+    const size_t es = s_double ? 16 : 8;
+    const int chunk = batched ? 4096 : 1;
+    for (int k0 = 0; k0 < nS; k0 += chunk) {
+      const int m = std::min(chunk, nS - k0);
+      const void* sk = static_cast<const char*>(s) + es * (size_t)k0 * stride_s;
+      const void* wk = w ? static_cast<const char*>(w) + es * (size_t)k0 * stride_w : nullptr;
+      const double* xk = xyz + (size_t)k0 * stride_xyz;
+      if (compute == BIPP_COMPUTE_FP32)
+        collect_impl<float>(ctx->eig32, m, wl + k0, s_double, sk, lds, stride_s, wk, ldw, stride_w,
+                            xk, ldxyz, stride_xyz);
+      else
+        collect_impl<double>(ctx->eig64, m, wl + k0, s_double, sk, lds, stride_s, wk, ldw,
+                             stride_w, xk, ldxyz, stride_xyz);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* bipp_last_error(void) { return g_last_error.c_str(); }
+
+const char* bipp_version(void) { return "1.0.0-mi355x"; }
+
+BippError bipp_device_count(int* count) {
+  return guard([&] {
+    require(count != nullptr, BIPP_INVALID_POINTER_ERROR, "null pointer");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+  });
+}
+
+BippError bipp_ctx_create(BippProcessingUnit pu, BippContext* ctx) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_POINTER_ERROR, "null pointer");
+    auto c = std::make_unique<BippContextImpl>();
+    c->impl = std::make_unique<ContextImpl>(pu);
+    *ctx = c.release();
+  });
+}
+
+BippError bipp_ctx_destroy(BippContext ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->impl->stream);
+    delete ctx;
+  });
+}
+
+BippError bipp_ctx_processing_unit(BippContext ctx, BippProcessingUnit* pu) {
+  return guard([&] {
+    require(ctx && pu, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    *pu = ctx->impl->pu;
+  });
+}
+
+BippError bipp_ctx_stream(BippContext ctx, void** stream) {
+  return guard([&] {
+    require(ctx && stream, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    *stream = ctx->impl->stream;
+  });
+}
+
+BippError bipp_ctx_synchronize(BippContext ctx) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    ctx->impl->sync();
+  });
+}
+
+BippError bipp_eigh_gram_batched(BippContext ctx, size_t nSamples, size_t nAntenna, size_t nBeam,
+                                 BippPrecision sPrec, BippPrecision compute, BippPrecision outPrec,
+                                 const double* wl, const void* s, size_t lds, size_t strideS,
+                                 const void* w, size_t ldw, size_t strideW, const double* xyz,
+                                 size_t ldxyz, size_t strideXyz, void* d, size_t strideD, void* v,
+                                 size_t ldv, size_t strideV, int32_t* nBeamReduced, void* scale) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null context");
+    require(s && xyz && wl && d, BIPP_INVALID_POINTER_ERROR, "null array");
+    if (nSamples == 0) return;
+    ContextImpl& c = *ctx->impl;
+    const bool sd = sPrec == BIPP_PRECISION_DOUBLE, od = outPrec == BIPP_PRECISION_DOUBLE;
+    if (compute == BIPP_PRECISION_DOUBLE) {
+      ctx->eig64.run(c, (int)nSamples, (int)nAntenna, (int)nBeam, wl, s, sd, lds, strideS, w, sd,
+                     ldw, strideW, xyz, ldxyz, strideXyz, false, 1.0);
+      export_eig<double>(c, ctx->eig64, (int)nSamples, (int)nAntenna, (int)nBeam, od, d, strideD, v,
+                         ldv, strideV, nBeamReduced, scale);
+      ctx->eig64.check(c);
+    } else {
+      ctx->eig32.run(c, (int)nSamples, (int)nAntenna, (int)nBeam, wl, s, sd, lds, strideS, w, sd,
+                     ldw, strideW, xyz, ldxyz, strideXyz, false, 1.0);
+      export_eig<float>(c, ctx->eig32, (int)nSamples, (int)nAntenna, (int)nBeam, od, d, strideD, v,
+                        ldv, strideV, nBeamReduced, scale);
+      ctx->eig32.check(c);
+    }
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared body of the single-sample host entry points (bipp::eigh_gram / bipp::eigh).
+template <typename T>
+void eigh_host(bool gram, T wl, size_t nAntenna, size_t nBeam, const void* s, size_t lds,
+               const void* w, size_t ldw, const T* xyz, size_t ldxyz, T* d, void* v, size_t ldv,
+               size_t* nBeamReduced, T* scale) {
+  require(s && d, BIPP_INVALID_POINTER_ERROR, "null array");
+  require(!gram || xyz, BIPP_INVALID_POINTER_ERROR, "null xyz");
+  require(nBeam > 0 && nAntenna > 0, BIPP_INVALID_PARAMETER_ERROR, "empty problem");
+  require(lds >= nBeam && (!w || ldw >= nAntenna) && (!v || ldv >= nAntenna) &&
+              (!gram || ldxyz >= nAntenna),
+          BIPP_INVALID_PARAMETER_ERROR, "leading dimension too small");
+  // The reference's eigh_gram takes no Context; a process-wide one serves these calls.  It
+  // is never destroyed, so no HIP call runs during static destruction.
+  static BippContextImpl* dflt = nullptr;
+  if (!dflt) {
+    auto c = std::make_unique<BippContextImpl>();
+    c->impl = std::make_unique<ContextImpl>(BIPP_PU_AUTO);
+    dflt = c.release();
+  }
+  ContextImpl& c = *dflt->impl;
+  hipStream_t st = c.stream;
+  constexpr bool dbl = sizeof(T) == 8;
+  const size_t ce = 2 * sizeof(T);
+  // the identity W of the reference's callers is passed explicitly; keep it general
+  std::vector<unsigned char> hs(ce * nBeam * nBeam);
+  for (size_t j = 0; j < nBeam; ++j)
+    std::memcpy(hs.data() + ce * j * nBeam, static_cast<const unsigned char*>(s) + ce * j * lds,
+                ce * nBeam);
+  DevBuf ds, dw, dxyz, dwl, dd, dv, dnr, dsc;
+  void* s_dev = ds.get(hs.size());
+  BIPP_HIP(hipMemcpyAsync(s_dev, hs.data(), hs.size(), hipMemcpyHostToDevice, st));
+  const void* w_dev = nullptr;
+  std::vector<unsigned char> hw;
+  if (w) {
+    hw.resize(ce * nAntenna * nBeam);
+    for (size_t j = 0; j < nBeam; ++j)
+      std::memcpy(hw.data() + ce * j * nAntenna, static_cast<const unsigned char*>(w) + ce * j * ldw,
+                  ce * nAntenna);
+    void* p = dw.get(hw.size());
+    BIPP_HIP(hipMemcpyAsync(p, hw.data(), hw.size(), hipMemcpyHostToDevice, st));
+    w_dev = p;
+  } else {
+    require(nAntenna == nBeam, BIPP_INVALID_PARAMETER_ERROR, "w == NULL needs nAntenna == nBeam");
+  }
+  std::vector<double> hx(3 * nAntenna, 0.0);
+  if (gram)
+    for (size_t c3 = 0; c3 < 3; ++c3)
+      for (size_t i = 0; i < nAntenna; ++i) hx[i + c3 * nAntenna] = static_cast<double>(xyz[i + c3 * ldxyz]);
+  double* xdev = dxyz.as<double>(3 * nAntenna);
+  BIPP_HIP(hipMemcpyAsync(xdev, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice, st));
+  double hwl = static_cast<double>(wl);
+  double* wldev = dwl.as<double>(1);
+  BIPP_HIP(hipMemcpyAsync(wldev, &hwl, sizeof(double), hipMemcpyHostToDevice, st));
+  T* d_dev = static_cast<T*>(dd.get(sizeof(T) * nBeam));
+  void* v_dev = dv.get(ce * nAntenna * nBeam);
+  int32_t* nr_dev = dnr.as<int32_t>(1);
+  T* sc_dev = static_cast<T*>(dsc.get(sizeof(T)));
+  if constexpr (dbl) {
+    dflt->eig64.run(c, 1, (int)nAntenna, (int)nBeam, wldev, s_dev, true, nBeam, 0, w_dev, true,
+                    nAntenna, 0, xdev, nAntenna, 0, !gram, 1.0);
+    export_eig<double>(c, dflt->eig64, 1, (int)nAntenna, (int)nBeam, true, d_dev, 0, v_dev,
+                       nAntenna, 0, nr_dev, sc_dev);
+    dflt->eig64.check(c);
+  } else {
+    dflt->eig32.run(c, 1, (int)nAntenna, (int)nBeam, wldev, s_dev, false, nBeam, 0, w_dev, false,
+                    nAntenna, 0, xdev, nAntenna, 0, !gram, 1.0);
+    export_eig<float>(c, dflt->eig32, 1, (int)nAntenna, (int)nBeam, false, d_dev, 0, v_dev,
+                      nAntenna, 0, nr_dev, sc_dev);
+    dflt->eig32.check(c);
+  }
+  std::vector<unsigned char> hv(ce * nAntenna * nBeam);
+  int32_t nr = 0;
+  T sc = 0;
+  BIPP_HIP(hipMemcpyAsync(d, d_dev, sizeof(T) * nBeam, hipMemcpyDeviceToHost, st));
+  BIPP_HIP(hipMemcpyAsync(hv.data(), v_dev, hv.size(), hipMemcpyDeviceToHost, st));
+  BIPP_HIP(hipMemcpyAsync(&nr, nr_dev, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  BIPP_HIP(hipMemcpyAsync(&sc, sc_dev, sizeof(T), hipMemcpyDeviceToHost, st));
+  BIPP_HIP(hipStreamSynchronize(st));
+  if (v)
+    for (size_t j = 0; j < nBeam; ++j)
+      std::memcpy(static_cast<unsigned char*>(v) + ce * j * ldv, hv.data() + ce * j * nAntenna,
+                  ce * nAntenna);
+  if (nBeamReduced) *nBeamReduced = static_cast<size_t>(nr);
+  if (scale) *scale = sc;
+}
+
+}  // namespace
+
+extern "C" {
+
+BippError bipp_eigh_gram_s(float wl, size_t nAntenna, size_t nBeam, const void* s, size_t lds,
+                           const void* w, size_t ldw, const float* xyz, size_t ldxyz, float* d,
+                           void* v, size_t ldv, size_t* nBeamReduced, float* scale) {
+  return guard([&] {
+    eigh_host<float>(true, wl, nAntenna, nBeam, s, lds, w, ldw, xyz, ldxyz, d, v, ldv, nBeamReduced,
+                     scale);
+  });
+}
+BippError bipp_eigh_gram_d(double wl, size_t nAntenna, size_t nBeam, const void* s, size_t lds,
+                           const void* w, size_t ldw, const double* xyz, size_t ldxyz, double* d,
+                           void* v, size_t ldv, size_t* nBeamReduced, double* scale) {
+  return guard([&] {
+    eigh_host<double>(true, wl, nAntenna, nBeam, s, lds, w, ldw, xyz, ldxyz, d, v, ldv,
+                      nBeamReduced, scale);
+  });
+}
+BippError bipp_eigh_s(float wl, size_t nAntenna, size_t nBeam, const void* s, size_t lds,
+                      const void* w, size_t ldw, float* d, void* v, size_t ldv,
+                      size_t* nBeamReduced, float* scale) {
+  return guard([&] {
+    eigh_host<float>(false, wl, nAntenna, nBeam, s, lds, w, ldw, nullptr, nAntenna, d, v, ldv,
+                     nBeamReduced, scale);
+  });
+}
+BippError bipp_eigh_d(double wl, size_t nAntenna, size_t nBeam, const void* s, size_t lds,
+                      const void* w, size_t ldw, double* d, void* v, size_t ldv,
+                      size_t* nBeamReduced, double* scale) {
+  return guard([&] {
+    eigh_host<double>(false, wl, nAntenna, nBeam, s, lds, w, ldw, nullptr, nAntenna, d, v, ldv,
+                      nBeamReduced, scale);
+  });
+}
+
+BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager, size_t nAntenna,
+                                size_t nBeam, size_t nLevels, const double* intervals,
+                                BippFilter filter, size_t nPixel, const float* lmn, size_t ldlmn,
+                                int batched, BippSynthesis* out) {
+  return guard([&] {
+    require(ctx && out, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(intervals && lmn, BIPP_INVALID_POINTER_ERROR, "null array");
+    require(nLevels >= 1 && nLevels <= 8, BIPP_INVALID_PARAMETER_ERROR, "1..8 levels supported");
+    require(nBeam >= 1 && nBeam <= 512, BIPP_INVALID_PARAMETER_ERROR, "1..512 beams supported");
+    require(imager == 0 || imager == 1, BIPP_INVALID_PARAMETER_ERROR, "unknown imager");
+    require(ldlmn >= nPixel, BIPP_INVALID_PARAMETER_ERROR, "ldlmn < nPixel");
+    require(compute == BIPP_COMPUTE_FP64 || compute == BIPP_COMPUTE_FP32 ||
+                compute == BIPP_COMPUTE_MIXED,
+            BIPP_INVALID_PARAMETER_ERROR, "unknown compute mode");
+    auto h = std::make_unique<BippSynthesisImpl>();
+    h->ctx = ctx;
+    h->compute = compute;
+    h->imager = imager;
+    h->n_ant = (int)nAntenna;
+    h->n_beam = (int)nBeam;
+    h->n_levels = (int)nLevels;
+    h->n_pix = (int)nPixel;
+    h->filter = filter;
+    h->batched = batched != 0;
+    hipStream_t st = ctx->impl->stream;
+    double* di = h->intervals.as<double>(2 * nLevels);
+    BIPP_HIP(hipMemcpyAsync(di, intervals, sizeof(double) * 2 * nLevels, hipMemcpyHostToDevice, st));
+    std::vector<double> l64(3 * nPixel);
+    std::vector<float> l32(3 * nPixel);
+    for (size_t c3 = 0; c3 < 3; ++c3)
+      for (size_t q = 0; q < nPixel; ++q) {
+        l32[c3 * nPixel + q] = lmn[q + c3 * ldlmn];
+        l64[c3 * nPixel + q] = static_cast<double>(lmn[q + c3 * ldlmn]);
+      }
+    double* d64 = h->lmn64.as<double>(l64.size());
+    float* d32 = h->lmn32.as<float>(l32.size());
+    BIPP_HIP(hipMemcpyAsync(d64, l64.data(), sizeof(double) * l64.size(), hipMemcpyHostToDevice, st));
+    BIPP_HIP(hipMemcpyAsync(d32, l32.data(), sizeof(float) * l32.size(), hipMemcpyHostToDevice, st));
+    const size_t isz = h->img_double() ? 8 : 4;
+    void* im = h->img.get(isz * nLevels * nPixel);
+    BIPP_HIP(hipMemsetAsync(im, 0, isz * nLevels * nPixel, st));
+    BIPP_HIP(hipStreamSynchronize(st));
+    *out = h.release();
+  });
+}
+
+BippError bipp_synthesis_destroy(BippSynthesis h) {
+  return guard([&] {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->ctx->impl->stream);
+    delete h;
+  });
+}
+
+BippError bipp_synthesis_collect(BippSynthesis h, size_t nSamples, const double* wl,
+                                 BippPrecision sPrec, const void* s, size_t lds, size_t strideS,
+                                 const void* w, size_t ldw, size_t strideW, const double* xyz,
+                                 size_t ldxyz, size_t strideXyz) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(s && xyz && wl, BIPP_INVALID_POINTER_ERROR, "null array");
+    if (nSamples == 0) return;
+    h->collect((int)nSamples, wl, sPrec == BIPP_PRECISION_DOUBLE, s, lds, strideS, w, ldw, strideW,
+               xyz, ldxyz, strideXyz);
+  });
+}
+
+BippError bipp_synthesis_collect_host(BippSynthesis h, size_t nSamples, const double* wl,
+                                      BippPrecision sPrec, const void* s, size_t lds,
+                                      size_t strideS, const void* w, size_t ldw, size_t strideW,
+                                      const double* xyz, size_t ldxyz, size_t strideXyz) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(s && xyz && wl, BIPP_INVALID_POINTER_ERROR, "null array");
+    if (nSamples == 0) return;
+    hipStream_t st = h->ctx->impl->stream;
+    const size_t es = sPrec == BIPP_PRECISION_DOUBLE ? 16 : 8;
+    const size_t sb = es * strideS * nSamples;
+    void* sd = h->st_s.get(sb);
+    BIPP_HIP(hipMemcpyAsync(sd, s, sb, hipMemcpyHostToDevice, st));
+    const void* wd = nullptr;
+    if (w) {
+      const size_t wb = es * strideW * nSamples;
+      void* p = h->st_w.get(wb);
+      BIPP_HIP(hipMemcpyAsync(p, w, wb, hipMemcpyHostToDevice, st));
+      wd = p;
+    }
+    const size_t xb = sizeof(double) * strideXyz * nSamples;
+    void* xd = h->st_xyz.get(xb);
+    BIPP_HIP(hipMemcpyAsync(xd, xyz, xb, hipMemcpyHostToDevice, st));
+    double* wld = h->st_wl.as<double>(nSamples);
+    BIPP_HIP(hipMemcpyAsync(wld, wl, sizeof(double) * nSamples, hipMemcpyHostToDevice, st));
+    h->collect((int)nSamples, wld, sPrec == BIPP_PRECISION_DOUBLE, sd, lds, strideS, wd, ldw,
+               strideW, static_cast<const double*>(xd), ldxyz, strideXyz);
+  });
+}
+
+BippError bipp_synthesis_get(BippSynthesis h, size_t level, int normalize, float* image) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(image != nullptr, BIPP_INVALID_POINTER_ERROR, "null image");
+    require(level < (size_t)h->n_levels, BIPP_INVALID_PARAMETER_ERROR, "level out of range");
+    hipStream_t st = h->ctx->impl->stream;
+    const double f = (normalize && h->n_collected) ? 1.0 / static_cast<double>(h->n_collected) : 1.0;
+    if (h->img_double()) {
+      std::vector<double> t(h->n_pix);
+      BIPP_HIP(hipMemcpyAsync(t.data(), static_cast<double*>(h->img.p) + level * h->n_pix,
+                              sizeof(double) * h->n_pix, hipMemcpyDeviceToHost, st));
+      BIPP_HIP(hipStreamSynchronize(st));
+      for (int q = 0; q < h->n_pix; ++q) image[q] = static_cast<float>(t[q] * f);
+    } else {
+      BIPP_HIP(hipMemcpyAsync(image, static_cast<float*>(h->img.p) + level * h->n_pix,
+                              sizeof(float) * h->n_pix, hipMemcpyDeviceToHost, st));
+      BIPP_HIP(hipStreamSynchronize(st));
+      const float ff = static_cast<float>(f);
+      if (normalize)
+        for (int q = 0; q < h->n_pix; ++q) image[q] *= ff;
+    }
+  });
+}
+
+BippError bipp_synthesis_image_device(BippSynthesis h, void** ptr, BippPrecision* type) {
+  return guard([&] {
+    require(h && ptr && type, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    *ptr = h->img.p;
+    *type = h->img_double() ? BIPP_PRECISION_DOUBLE : BIPP_PRECISION_SINGLE;
+  });
+}
+
+BippError bipp_synthesis_num_collected(BippSynthesis h, size_t* n) {
+  return guard([&] {
+    require(h && n, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    *n = h->n_collected;
+  });
+}
+
+BippError bipp_synthesis_reset(BippSynthesis h) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    hipStream_t st = h->ctx->impl->stream;
+    const size_t isz = h->img_double() ? 8 : 4;
+    BIPP_HIP(hipMemsetAsync(h->img.p, 0, isz * h->n_levels * h->n_pix, st));
+    BIPP_HIP(hipStreamSynchronize(st));
+    h->n_collected = 0;
+  });
+}
+
+BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, double* d, int32_t* nBeamReduced) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    if (d) std::copy(h->last_d.begin(), h->last_d.end(), d);
+    if (nBeamReduced) std::copy(h->last_nred.begin(), h->last_nred.end(), nBeamReduced);
+  });
+}
+
+BippError bipp_virtual_vis(BippContext ctx, BippPrecision prec, size_t nSamples, size_t nAntenna,
+                           size_t nEig, const void* v, size_t ldv, size_t strideV, const void* dw,
+                           size_t strideDw, void* vv, size_t strideVv) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null context");
+    require(v && dw && vv, BIPP_INVALID_POINTER_ERROR, "null array");
+    hipStream_t st = ctx->impl->stream;
+    if (prec == BIPP_PRECISION_DOUBLE)
+      launch_virtual_vis<double>((int)nSamples, (int)nAntenna, (int)nEig,
+                                 static_cast<const cplx<double>*>(v), ldv, strideV,
+                                 static_cast<const double*>(dw), strideDw,
+                                 static_cast<cplx<double>*>(vv), strideVv, st);
+    else
+      launch_virtual_vis<float>((int)nSamples, (int)nAntenna, (int)nEig,
+                                static_cast<const cplx<float>*>(v), ldv, strideV,
+                                static_cast<const float*>(dw), strideDw,
+                                static_cast<cplx<float>*>(vv), strideVv, st);
+  });
+}
+
+BippError bipp_nuft_sum(BippContext ctx, BippPrecision prec, double alpha, size_t nIn,
+                        const void* input, const void* u, const void* v, const void* w,
+                        size_t nOut, const void* x, const void* y, const void* z, void* out) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null context");
+    require(input && u && v && w && x && y && z && out, BIPP_INVALID_POINTER_ERROR, "null array");
+    hipStream_t st = ctx->impl->stream;
+    if (prec == BIPP_PRECISION_DOUBLE)
+      launch_nuft_sum<double>(alpha, (int)nIn, static_cast<const cplx<double>*>(input),
+                              static_cast<const double*>(u), static_cast<const double*>(v),
+                              static_cast<const double*>(w), (int)nOut,
+                              static_cast<const double*>(x), static_cast<const double*>(y),
+                              static_cast<const double*>(z), static_cast<double*>(out), st);
+    else
+      launch_nuft_sum<float>(static_cast<float>(alpha), (int)nIn,
+                             static_cast<const cplx<float>*>(input), static_cast<const float*>(u),
+                             static_cast<const float*>(v), static_cast<const float*>(w), (int)nOut,
+                             static_cast<const float*>(x), static_cast<const float*>(y),
+                             static_cast<const float*>(z), static_cast<float*>(out), st);
+  });
+}
+
+BippError bipp_nuft_sum_mixed(BippContext ctx, double alpha, size_t nIn, const void* input,
+                              const double* u, const double* v, const double* w, size_t nOut,
+                              const double* x, const double* y, const double* z, float* out) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null context");
+    require(input && u && v && w && x && y && z && out, BIPP_INVALID_POINTER_ERROR, "null array");
+    launch_nuft_sum_mixed(alpha, (int)nIn, static_cast<const cplx<float>*>(input), u, v, w,
+                          (int)nOut, x, y, z, out, ctx->impl->stream);
+  });
+}
+
+BippError bipp_gram(BippContext ctx, BippPrecision prec, size_t nSamples, size_t nAntenna,
+                    const double* wl, const double* xyz, size_t ldxyz, size_t strideXyz,
+                    double prefactor, void* g, size_t ldg, size_t strideG) {
+  return guard([&] {
+    require(ctx != nullptr, BIPP_INVALID_HANDLE_ERROR, "null context");
+    require(wl && xyz && g, BIPP_INVALID_POINTER_ERROR, "null array");
+    hipStream_t st = ctx->impl->stream;
+    if (prec == BIPP_PRECISION_DOUBLE)
+      launch_sinc_matrix<double>((int)nSamples, (int)nAntenna, xyz, ldxyz, strideXyz, wl, prefactor,
+                                 static_cast<cplx<double>*>(g), ldg, strideG, st);
+    else
+      launch_sinc_matrix<float>((int)nSamples, (int)nAntenna, xyz, ldxyz, strideXyz, wl, prefactor,
+                                static_cast<cplx<float>*>(g), ldg, strideG, st);
+  });
+}
+
+}  // extern "C"

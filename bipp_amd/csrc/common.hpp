@@ -1,0 +1,116 @@
+// Shared device/host helpers for the gfx950 Bluebild kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "bipp/enums.h"
+#include "bipp/errors.h"
+
+namespace bippamd {
+
+// ---------------------------------------------------------------------------------------
+// Errors: every failure inside the library is a BippException carrying a BippError code;
+// the C-ABI layer converts it to a return code plus a thread-local message.
+// ---------------------------------------------------------------------------------------
+struct BippException : std::runtime_error {
+  BippError code;
+  BippException(BippError c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+[[noreturn]] inline void fail(BippError c, const std::string& msg) { throw BippException(c, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    fail(BIPP_GPU_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define BIPP_HIP(call) ::bippamd::hip_check((call), #call)
+
+inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+// ---------------------------------------------------------------------------------------
+// Complex arithmetic on plain structs (layout-compatible with std::complex<R>).
+// ---------------------------------------------------------------------------------------
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R re, im;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> mk(R re, R im) {
+  cplx<R> c;
+  c.re = re;
+  c.im = im;
+  return c;
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) {
+  return mk(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) {
+  return mk(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, R s) {
+  return mk(a.re * s, a.im * s);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> conj(cplx<R> a) {
+  return mk(a.re, -a.im);
+}
+// a * conj(b)
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> mulc(cplx<R> a, cplx<R> b) {
+  return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+// conj(a) * b
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> cmul_conj_a(cplx<R> a, cplx<R> b) {
+  return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+template <typename R>
+__host__ __device__ __forceinline__ R abs2(cplx<R> a) {
+  return a.re * a.re + a.im * a.im;
+}
+template <typename R, typename S>
+__host__ __device__ __forceinline__ cplx<R> cvt(cplx<S> a) {
+  return mk<R>(static_cast<R>(a.re), static_cast<R>(a.im));
+}
+
+// ---------------------------------------------------------------------------------------
+// Wave64 / workgroup reductions.
+// ---------------------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over a workgroup of NT threads; `scratch` holds NT/64 values.  All threads get
+// the result.  Contains two barriers.
+template <int NT, typename R>
+__device__ __forceinline__ R block_sum(R v, R* scratch) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  R s = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += scratch[w];
+  __syncthreads();
+  return s;
+}
+
+inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+}  // namespace bippamd
