@@ -1,0 +1,225 @@
+"""Benchmark: (timestep, channel) samples imaged per second on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
+
+A step is one pass of the fused Bluebild chain over the configuration's whole batch of
+synthetic samples (flag -> Gram -> generalized eigensolve -> energy levels -> image),
+with the samples already resident in HBM.  `value` counts samples per second of the
+mixed/batched form (float64 eigensolve and phase reduction, split-bf16 MFMA contraction),
+valid only when every level image is within 1e-4 relative L2 of the float64 GPU run of
+the same inputs and the level masks are identical.  The plain float32 per-sample form is
+timed beside it, and the CPU restatement of the reference path is timed on the host.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import bipp_amd  # noqa: E402
+from bipp_amd import hip, synthetic  # noqa: E402
+from bipp_amd.hip import DeviceArray  # noqa: E402
+
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def upload(wl_: synthetic.Workload):
+    s, xyz, wl = synthetic.stack_samples(wl_.samples, np.complex64)
+    return DeviceArray.from_numpy(s), DeviceArray.from_numpy(xyz), DeviceArray.from_numpy(wl)
+
+
+def level_intervals(ctx, dev, n_s, n, n_levels):
+    """Eigenvalues of every sample (GPU, float64) -> KMeans intervals, as the reference's
+    parameter estimation does before synthesis (python/bipp/parameter_estimator.py:55-105)."""
+    s_d, x_d, wl_d = dev
+    d_d = DeviceArray.zeros((n_s, n), np.float64)
+    nr_d = DeviceArray.zeros(n_s, np.int32)
+    sc_d = DeviceArray.zeros(n_s, np.float64)
+    bipp_amd.call("bipp_eigh_gram_batched", ctx.handle, n_s, n, n, 0, 1, 1, wl_d.ptr, s_d.ptr, n,
+                  n * n, None, 0, 0, x_d.ptr, n, 3 * n, d_d.ptr, n, None, n, n * n, nr_d.ptr,
+                  sc_d.ptr)
+    ctx.synchronize()
+    d = d_d.numpy()
+    return bipp_amd.infer_intervals(n_levels, 1.0, "log", 0.0, float("inf"), d), d
+
+
+def eig_count(d, intervals):
+    """Eigenvectors with a nonzero weight in some level (rows of the contraction)."""
+    df = d.astype(np.float32)
+    m = np.zeros(df.shape, bool)
+    for lo, hi in intervals:
+        m |= (df >= np.float32(lo)) & (df <= np.float32(hi)) & (df != 0)
+    return m.sum(axis=1)
+
+
+def run_form(ctx, wl_, dev, intervals, compute, batched, steps, warmup):
+    s_d, x_d, wl_d = dev
+    n_s, n = wl_.n_samples, wl_.n_antenna
+    syn = bipp_amd.NufftSynthesis(ctx, n, n, intervals, wl_.lmn, compute=compute,
+                                  imager="steering", batched=batched)
+    for _ in range(warmup):
+        syn.reset()
+        syn.collect_device(n_s, wl_d.ptr, s_d.ptr, False, x_d.ptr)
+    hip.synchronize()
+    syn.timings(reset=True)
+    times = []
+    for _ in range(steps):
+        syn.reset()
+        hip.synchronize()
+        t0 = time.perf_counter()
+        syn.collect_device(n_s, wl_d.ptr, s_d.ptr, False, x_d.ptr)
+        hip.synchronize()
+        times.append(time.perf_counter() - t0)
+    tm = syn.timings()
+    return syn, float(np.mean(times)), tm
+
+
+def cpu_baseline(wl_, intervals, n_samples=2, grid=96):
+    """CPU restatement of the reference path (float32 chegv + closed-form image, OpenMP),
+    on `n_samples` samples and a grid x grid pixel subset, scaled to the full grid."""
+    from oracle import cpu as ocpu
+
+    L = ocpu.lib()
+    cores = L.bippc_max_threads()
+    lm_full = wl_.lmn
+    sub = synthetic.make_lmn_grid(grid, math.radians(wl_.fov_deg))
+    t_eig = t_img = 0.0
+    for smp in wl_.samples[:n_samples]:
+        L.bippc_lapack_threads(cores)
+        t0 = time.perf_counter()
+        v, d, nr, sc = ocpu.eigh_gram_c(smp.wl, smp.s, smp.xyz)
+        t_eig += time.perf_counter() - t0
+        df = d.astype(np.float32)
+        w = np.stack([sc * df * ((df >= np.float32(lo)) & (df <= np.float32(hi)))
+                      for lo, hi in intervals]).astype(np.float32)
+        keep = np.abs(w).sum(0) > 0
+        img = np.zeros((len(intervals), sub.shape[0]), np.float32)
+        t0 = time.perf_counter()
+        ocpu.steering_image(v[:, keep], w[:, keep], smp.xyz / smp.wl, sub, img)
+        t_img += time.perf_counter() - t0
+    per_sample = (t_eig + t_img * lm_full.shape[0] / sub.shape[0]) / n_samples
+    return {"value": 1.0 / per_sample, "unit": "samples/s", "cores": int(cores), "kind": "port",
+            "sample": (f"{n_samples} samples: float32 chegv eigensolve (scipy OpenBLAS, {cores} "
+                       f"threads) + closed-form image on a {grid}x{grid} pixel subset, image time "
+                       f"scaled by {lm_full.shape[0] // sub.shape[0]}x to the full grid; the "
+                       f"reference's neonufft CPU path cannot be built offline"),
+            "eig_s_per_sample": t_eig / n_samples,
+            "image_s_per_sample_full_grid": t_img * lm_full.shape[0] / sub.shape[0] / n_samples}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--timesteps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the sample-parallel "
+              f"multi-GPU form is described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    wl_ = synthetic.make_workload(a.config, n_timesteps=a.timesteps)
+    n_s, n, P = wl_.n_samples, wl_.n_antenna, wl_.lmn.shape[0]
+    log(f"config {a.config}: {n_s} samples, {n} antennas, {P} pixels, {wl_.n_levels} levels "
+        f"(setup {time.perf_counter() - t_setup:.1f}s)")
+    ctx = bipp_amd.Context("GPU")
+    dev = upload(wl_)
+    intervals, d_all = level_intervals(ctx, dev, n_s, n, wl_.n_levels)
+    k_eig = eig_count(d_all, intervals)
+    flops = float(np.sum(8.0 * n * k_eig * P))   # complex GEMM V^T E per sample: 8 N K P
+
+    # mixed / batched form (the measured value)
+    syn_m, t_m, tm_m = run_form(ctx, wl_, dev, intervals, "mixed", True, a.steps, a.warmup)
+    img_m = syn_m.images()
+    d_m, _ = syn_m.last_eigenvalues(n_s)
+    log(f"mixed batched: {t_m * 1e3:.1f} ms/step  stages {tm_m}")
+
+    # float64 GPU oracle of the same inputs
+    syn_o, t_o, _ = run_form(ctx, wl_, dev, intervals, "fp64", True, 1, 0)
+    img_o = syn_o.images()
+    d_o, _ = syn_o.last_eigenvalues(n_s)
+    errs_m = [float(np.linalg.norm(img_m[l] - img_o[l]) / np.linalg.norm(img_o[l]))
+              for l in range(len(intervals))]
+    masks_equal = bool(np.array_equal(eig_count_mask(d_m, intervals), eig_count_mask(d_o, intervals)))
+    log(f"fp64 oracle run {t_o * 1e3:.1f} ms; mixed rel L2 per level {errs_m}; masks equal {masks_equal}")
+
+    plain = None
+    if not a.no_plain:
+        syn_p, t_p, tm_p = run_form(ctx, wl_, dev, intervals, "fp32", False, 1, 1)
+        img_p = syn_p.images()
+        errs_p = [float(np.linalg.norm(img_p[l] - img_o[l]) / np.linalg.norm(img_o[l]))
+                  for l in range(len(intervals))]
+        plain = {"samples_per_s": n_s / t_p, "ms_per_step": t_p * 1e3, "max_rel_err": max(errs_p),
+                 "form": "float32 everywhere, one launch sequence per sample"}
+        log(f"plain fp32 per-sample: {t_p * 1e3:.1f} ms/step, rel err {errs_p}")
+
+    ok = max(errs_m) <= 1e-4 and masks_equal
+    value = n_s / t_m if ok else 0.0
+    kern_ms = tm_m["image_ms"] / max(tm_m["batches"], 1)
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(wl_, intervals)
+            log(f"cpu baseline {cpu}")
+        except Exception as exc:  # the CPU leg must not hide the GPU result
+            log(f"cpu baseline failed: {exc}")
+    out = {
+        "metric": "timesteps*channels imaged/s at rel err <=1e-4 vs FP64 (mixed/batched)",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_m * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3-mfma/fp32-acc, fp64 eigensolve",
+        "data": "synthetic (seeded disk array, point-source sky, noise 0.1)",
+        "config": {"workload": wl_.name, "samples": n_s, "antennas": n, "pixels": P,
+                   "levels": len(intervals), "imager": "steering", "batch": n_s},
+        "accuracy": {"max_rel_l2_vs_fp64": max(errs_m), "per_level": errs_m,
+                     "level_masks_identical": masks_equal},
+        "plain_fp32": plain,
+        "speedup_vs_plain": (value / plain["samples_per_s"]) if (plain and value) else None,
+        "stages_ms_per_step": {k: v / max(tm_m["batches"], 1) for k, v in tm_m.items()
+                               if k != "batches"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "steer_kernel<BF16X3>",
+                     "algorithmic_flops_per_launch": flops,
+                     "mfma_flops_executed_per_launch": 3 * flops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def eig_count_mask(d, intervals):
+    df = d.astype(np.float32)
+    return np.stack([(df >= np.float32(lo)) & (df <= np.float32(hi)) & (df != 0)
+                     for lo, hi in intervals])
+
+
+if __name__ == "__main__":
+    main()

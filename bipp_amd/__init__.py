@@ -238,7 +238,7 @@ class NufftSynthesis:
 
     def __init__(self, ctx: Context, n_antenna: int, n_beam: int, intervals, lmn,
                  filter_name: str = "lsq", compute: str = "mixed", imager: str = "steering",
-                 batched: bool = True) -> None:
+                 batched: bool = True, tolerance: Optional[float] = None) -> None:
         self.ctx = ctx
         iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64).reshape(-1, 2))
         lmn = np.asfortranarray(np.asarray(lmn, dtype=np.float32))
@@ -254,6 +254,13 @@ class NufftSynthesis:
             self.n_pixel, lmn.ctypes.data_as(C.POINTER(C.c_float)), self.n_pixel,
             1 if batched else 0, C.byref(h)))
         self._h = h
+        if tolerance is not None:
+            check(lib().bipp_synthesis_set_tolerance(h, float(tolerance)))
+
+    def nufft_grid(self) -> Tuple[int, int, int]:
+        nf = (C.c_int32 * 3)()
+        check(lib().bipp_synthesis_nufft_grid(self._h, nf))
+        return tuple(nf)
 
     def collect(self, wl, s, xyz, w=None) -> None:
         """Host arrays: wl (nS,) or scalar; s (nS, nBeam, nBeam) or (nBeam, nBeam), matrices
@@ -316,6 +323,13 @@ class NufftSynthesis:
 
     def reset(self) -> None:
         check(lib().bipp_synthesis_reset(self._h))
+
+    def timings(self, reset: bool = False) -> Dict[str, float]:
+        """GPU milliseconds per stage accumulated over collect calls (HIP events)."""
+        ms = np.zeros(4)
+        check(lib().bipp_synthesis_timings(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                           1 if reset else 0))
+        return {"eigensolve_ms": ms[0], "prep_ms": ms[1], "image_ms": ms[2], "batches": ms[3]}
 
     def last_eigenvalues(self, n_samples: int):
         d = np.zeros((n_samples, self.n_beam), dtype=np.float64)

@@ -29,7 +29,7 @@ PU = {"AUTO": 0, "CPU": 1, "GPU": 2}
 PRECISION = {"single": 0, "double": 1}
 COMPUTE = {"fp64": 0, "fp32": 1, "mixed": 2}
 FILTERS = {"lsq": 0, "std": 1, "sqrt": 2, "inv": 3, "inv_sqrt": 4}
-IMAGERS = {"steering": 0, "direct": 1}
+IMAGERS = {"steering": 0, "direct": 1, "nufft": 2}
 
 
 class GenericError(RuntimeError):
@@ -86,6 +86,9 @@ _SIGS = {
     "bipp_synthesis_num_collected": (C.c_int, [vp, C.POINTER(sz)]),
     "bipp_synthesis_reset": (C.c_int, [vp]),
     "bipp_synthesis_last_eigenvalues": (C.c_int, [vp, dp, C.POINTER(i32)]),
+    "bipp_synthesis_timings": (C.c_int, [vp, dp, C.c_int]),
+    "bipp_synthesis_set_tolerance": (C.c_int, [vp, C.c_double]),
+    "bipp_synthesis_nufft_grid": (C.c_int, [vp, C.POINTER(i32)]),
     "bipp_virtual_vis": (C.c_int, [vp, C.c_int, sz, sz, sz, vp, sz, sz, vp, sz, vp, sz]),
     "bipp_nuft_sum": (C.c_int, [vp, C.c_int, C.c_double, sz, vp, vp, vp, vp, sz, vp, vp, vp,
                                 vp]),

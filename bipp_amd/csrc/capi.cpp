@@ -3,15 +3,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "bipp_hip.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "nufft3.hpp"
 
 namespace bippamd {
 
@@ -165,18 +169,22 @@ struct EigRunner {
     launch_flag(fp, nS, s_double, st);
 
     const cplx<R>* g_in = nullptr;
-    if (general_w && !standard) {
-      // G_r = W_I^H Sigma W_I (src/host/gram_matrix.cpp:58-61)
-      cplx<R>* sg = sig.as<cplx<R>>((size_t)nS * nAnt * nAnt);
-      launch_sinc_matrix<R>(nS, nAnt, xyz, ldxyz, stride_xyz, wl, prefactor, sg, nAnt,
-                            (long long)nAnt * nAnt, st);
-      cplx<R>* wi = w_i.as<cplx<R>>((size_t)nS * nAnt * nmax);
+    cplx<R>* wi = nullptr;
+    if (general_w) {
+      // W_I = W[:, kept beams] (src/eigensolver.cpp:201-203)
+      wi = w_i.as<cplx<R>>((size_t)nS * nAnt * nmax);
       if (w_double)
         launch_gather_w<R, double>(nS, nAnt, nBeam, w, ldw, stride_w, fp.n_red, fp.idx, nmax, wi,
                                    nAnt, (long long)nAnt * nmax, st);
       else
         launch_gather_w<R, float>(nS, nAnt, nBeam, w, ldw, stride_w, fp.n_red, fp.idx, nmax, wi,
                                   nAnt, (long long)nAnt * nmax, st);
+    }
+    if (general_w && !standard) {
+      // G_r = W_I^H Sigma W_I (src/host/gram_matrix.cpp:58-61)
+      cplx<R>* sg = sig.as<cplx<R>>((size_t)nS * nAnt * nAnt);
+      launch_sinc_matrix<R>(nS, nAnt, xyz, ldxyz, stride_xyz, wl, prefactor, sg, nAnt,
+                            (long long)nAnt * nAnt, st);
       cplx<R>* t = tmp.as<cplx<R>>((size_t)nS * nAnt * nmax);
       GemmBatch<R> g1{};
       g1.n_batch = nS;
@@ -353,11 +361,375 @@ void export_eig(ContextImpl& ctx, EigRunner<R>& er, int nS, int nAnt, int nBeam,
 
 using namespace bippamd;
 
+// A context is shared by the synthesis handles created from it: each handle holds a
+// reference, so destroying the context first leaves it alive until its last handle goes.
 struct BippContextImpl {
   std::unique_ptr<ContextImpl> impl;
   EigRunner<float> eig32;
   EigRunner<double> eig64;
+  std::atomic<int> refs{1};
+
+  void retain() { refs.fetch_add(1); }
+  void release() {
+    if (refs.fetch_sub(1) == 1) {
+      (void)hipStreamSynchronize(impl->stream);
+      delete this;
+    }
+  }
 };
+
+// ---------------------------------------------------------------------------------------
+// Type-3 NUFFT plan (host): kernel, grid sizes, FFT factorisation, correction tables.
+// Kernel width and shape follow the published exponential-of-semicircle rule for
+// upsampling factor 2 (Barnett, Magland & af Klinteberg 2019): w = ceil(log10(1/tol)) + 1,
+// beta = 2.30 w (2.20, 2.26, 2.38 for w = 2, 3, 4); grid size per dimension
+// nf1 = smooth(2 sigma / pi * X * S + w + 1) and an inner type-2 grid nf2 = smooth(2 nf1).
+// ---------------------------------------------------------------------------------------
+namespace bippamd {
+namespace {
+
+int next_smooth(double x) {
+  long long n = static_cast<long long>(std::ceil(x));
+  if (n < 2) n = 2;
+  n += n & 1;
+  for (;; n += 2) {
+    long long m = n;
+    for (int p : {2, 3, 5})
+      while (m % p == 0) m /= p;
+    if (m == 1) return static_cast<int>(n);
+  }
+}
+
+N3Radix radices_of(int m) {
+  N3Radix r;
+  while (m % 4 == 0) {
+    r.r[r.count++] = 4;
+    m /= 4;
+  }
+  for (int p : {2, 3, 5})
+    while (m % p == 0) {
+      r.r[r.count++] = p;
+      m /= p;
+    }
+  if (m != 1 || r.count > 20) fail(BIPP_INTERNAL_ERROR, "FFT length is not 2^a 3^b 5^c");
+  return r;
+}
+
+// n = n1 * n2 with n1, n2 <= 128 (n2 == 1 when n <= 128 itself)
+void split_axis(int n, int& n1, int& n2) {
+  if (n <= 128) {
+    n1 = n;
+    n2 = 1;
+    return;
+  }
+  for (int a = 128; a >= 2; --a)
+    if (n % a == 0 && n / a <= 128) {
+      n1 = a;
+      n2 = n / a;
+      return;
+    }
+  fail(BIPP_INVALID_PARAMETER_ERROR,
+       "NUFFT grid axis of " + std::to_string(n) + " points exceeds the single-plan limit (16384)");
+}
+
+// Gauss-Legendre nodes/weights mapped to [0, 1] (Newton iteration on P_n)
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+    double dp = 1;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1 - z);
+    w[i] = 1.0 / ((1 - z * z) * dp * dp);  // 2/((1-z^2) P'^2) scaled by 1/2 for [0, 1]
+  }
+}
+
+double es_ft_host(double xi, double beta, const std::vector<double>& x, const std::vector<double>& w) {
+  double s = 0;
+  for (size_t i = 0; i < x.size(); ++i)
+    s += w[i] * std::exp(beta * (std::sqrt(1 - x[i] * x[i]) - 1)) * std::cos(xi * x[i]);
+  return 2 * s;
+}
+
+}  // namespace
+
+struct N3Plan {
+  int w = 6;
+  double beta = 13.8;
+  int nf1[3], nf2[3], n1[3], n2[3];
+  double h1[3], h2[3], gam[3], C[3], D[3];
+  int bin[3] = {8, 8, 8}, nbins[3], off[3];
+  long long grid_cells = 0;
+  std::vector<double> gl_x, gl_w;
+  std::vector<double> deconv[3];
+
+  void make(double tol, const double X_in[3], const double S_in[3], const double C_in[3],
+            const double D_in[3], int max_x) {
+    w = static_cast<int>(std::ceil(std::log10(1.0 / tol))) + 1;
+    w = std::max(2, std::min(w, 16));
+    beta = w == 2 ? 2.20 * w : (w == 3 ? 2.26 * w : (w == 4 ? 2.38 * w : 2.30 * w));
+    const double sigma = 2.0;
+    gauss_legendre01(64, gl_x, gl_w);
+    grid_cells = 1;
+    for (int d = 0; d < 3; ++d) {
+      const double S = std::max(S_in[d], 1e-9);
+      const double X = std::max(X_in[d], 1e-9);
+      nf1[d] = std::max(next_smooth(2 * sigma / 3.14159265358979323846 * X * S + w + 1), 2 * w);
+      nf2[d] = next_smooth(2.0 * nf1[d]);
+      h1[d] = 2 * 3.14159265358979323846 / nf1[d];
+      h2[d] = 2 * 3.14159265358979323846 / nf2[d];
+      gam[d] = nf1[d] / (2 * sigma * S);
+      C[d] = C_in[d];
+      D[d] = D_in[d];
+      if (d == 0) {
+        if (nf2[0] > max_x)
+          fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT grid x axis of " + std::to_string(nf2[0]) +
+                                                 " points exceeds the single-plan limit");
+        n1[0] = nf2[0];
+        n2[0] = 1;
+      } else {
+        split_axis(nf2[d], n1[d], n2[d]);
+      }
+      off[d] = nf1[d] / 2 + w;
+      nbins[d] = (nf1[d] + 2 * w) / bin[d] + 1;
+      grid_cells *= nf2[d];
+      deconv[d].resize(nf1[d]);
+      const double a2 = 0.5 * w * h2[d];
+      for (int i = 0; i < nf1[d]; ++i) {
+        const int l = i - nf1[d] / 2;
+        deconv[d][i] = 1.0 / (a2 * es_ft_host(l * a2, beta, gl_x, gl_w));
+      }
+    }
+  }
+  int n_bins() const { return nbins[0] * nbins[1] * nbins[2]; }
+};
+
+// Runs the NUFFT image stage for one batch: points from positions, values from
+// VV_l = Z diag(w_l) Z^H, all levels accumulated into `img` (TI per pixel).
+template <typename TG>
+struct N3Runner {
+  DevBuf grid, valid, bin_of, l0x, l0y, l0z, frx, fry, frz, phr, phi, mult, src;
+  DevBuf bin_count, bin_start, bin_fill, scan_tmp, sorted, vv, deconv, gl;
+
+  template <typename RE>
+  void run(ContextImpl& c, const N3Plan& pl, int nS, int L, int n_cols, int rows_max,
+           const int* n_rows, const cplx<RE>* z, long long stride_z, int ldz, const RE* wd,
+           const double* pos, long long pos_stride, int apad, const double* lmn64, int n_pix,
+           void* img, bool phase_fp32) {
+    hipStream_t st = c.stream;
+    // virtual visibilities of every (sample, level), lower triangle
+    const long long ldv = rows_max;
+    const long long vv_stride = ldv * ldv;
+    cplx<TG>* vvp = vv.as<cplx<TG>>((size_t)nS * L * vv_stride);
+    N3VVParams<RE> vp;
+    vp.n_levels = L;
+    vp.n_cols = n_cols;
+    vp.n_rows = n_rows;
+    vp.n_rows_const = rows_max;
+    vp.z = z;
+    vp.stride_z = stride_z;
+    vp.ldz = ldz;
+    vp.wd = wd;
+    vp.vv = vvp;
+    vp.vv_stride = vv_stride;
+    vp.ld_vv = ldv;
+    launch_vv_levels<RE, TG>(vp, nS, rows_max, st);
+
+    // points, bins, sort
+    const long long npair = (long long)rows_max * (rows_max + 1) / 2;
+    const long long npts = npair * nS;
+    N3PointsParams pp;
+    pp.n_ant_max = rows_max;
+    pp.n_rows = n_rows;
+    pp.n_rows_const = rows_max;
+    pp.pos = pos;
+    pp.pos_stride = pos_stride;
+    pp.apad = apad;
+    for (int d = 0; d < 3; ++d) {
+      pp.C[d] = pl.C[d];
+      pp.D[d] = pl.D[d];
+      pp.inv_gh[d] = 1.0 / (pl.gam[d] * pl.h1[d]);
+      pp.off[d] = pl.off[d];
+      pp.bin_size[d] = pl.bin[d];
+      pp.nbins[d] = pl.nbins[d];
+    }
+    pp.w = pl.w;
+    pp.phase_fp32 = phase_fp32 ? 1 : 0;
+    pp.ld_vv = ldv;
+    pp.valid = valid.as<unsigned char>(npts);
+    pp.bin_of = bin_of.as<int>(npts);
+    pp.l0x = l0x.as<int>(npts);
+    pp.l0y = l0y.as<int>(npts);
+    pp.l0z = l0z.as<int>(npts);
+    pp.frx = frx.as<float>(npts);
+    pp.fry = fry.as<float>(npts);
+    pp.frz = frz.as<float>(npts);
+    pp.phr = phr.as<float>(npts);
+    pp.phi = phi.as<float>(npts);
+    pp.mult = mult.as<float>(npts);
+    pp.src = src.as<int>(npts);
+    const int nb = pl.n_bins();
+    pp.bin_count = bin_count.as<unsigned>(nb);
+    launch_zero<unsigned>(pp.bin_count, nb, st);
+    launch_n3_points<double>(pp, nS, st);
+    unsigned* bstart = bin_start.as<unsigned>(nb);
+    unsigned* btmp = scan_tmp.as<unsigned>(3 * ((size_t)nb / 1024 + 2) + 4096);
+    launch_scan(pp.bin_count, bstart, btmp, nb, st);
+    unsigned* bfill = bin_fill.as<unsigned>(nb);
+    launch_zero<unsigned>(bfill, nb, st);
+    unsigned* srt = sorted.as<unsigned>(npts);
+    launch_n3_scatter(npts, pp.valid, pp.bin_of, bstart, bfill, srt, st);
+
+    // grid: zero, spread, deconvolve
+    const long long G = pl.grid_cells;
+    TG* gp = grid.as<TG>((size_t)2 * L * G);
+    launch_zero<TG>(gp, 2LL * L * G, st);
+    N3SpreadParams<TG> sp;
+    sp.n_levels = L;
+    sp.w = pl.w;
+    sp.beta = pl.beta;
+    for (int d = 0; d < 3; ++d) {
+      sp.bin_size[d] = pl.bin[d];
+      sp.nbins[d] = pl.nbins[d];
+      sp.off[d] = pl.off[d];
+      sp.nf2[d] = pl.nf2[d];
+    }
+    sp.bin_start = bstart;
+    sp.bin_count = pp.bin_count;
+    sp.sorted = srt;
+    sp.l0x = pp.l0x;
+    sp.l0y = pp.l0y;
+    sp.l0z = pp.l0z;
+    sp.src = pp.src;
+    sp.frx = pp.frx;
+    sp.fry = pp.fry;
+    sp.frz = pp.frz;
+    sp.phr = pp.phr;
+    sp.phi = pp.phi;
+    sp.mult = pp.mult;
+    sp.npair = npair;
+    sp.vv = vvp;
+    sp.vv_stride = vv_stride;
+    sp.grid = gp;
+    sp.grid_stride = G;
+    launch_n3_spread<TG, cplx<TG>>(sp, nb, st);
+
+    const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
+    std::vector<TG> hdec(nd);
+    for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
+    for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
+    for (int i = 0; i < pl.nf1[2]; ++i)
+      hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
+    TG* ddec = deconv.as<TG>(nd);
+    BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
+    N3DeconvParams<TG> dp;
+    for (int d = 0; d < 3; ++d) {
+      dp.nf1[d] = pl.nf1[d];
+      dp.nf2[d] = pl.nf2[d];
+    }
+    dp.fx = ddec;
+    dp.fy = ddec + pl.nf1[0];
+    dp.fz = ddec + pl.nf1[0] + pl.nf1[1];
+    dp.grid = gp;
+    dp.grid_stride = G;
+    launch_n3_deconv<TG>(dp, L, st);
+
+    // FFT (sign +) along x, y, z
+    cplx<TG>* cg = reinterpret_cast<cplx<TG>*>(gp);
+    const long long nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
+    launch_fft_x<TG>(cg, (int)nx, (long long)L * ny * nz, radices_of((int)nx), st);
+    for (int d = 1; d < 3; ++d) {
+      N3StridedParams<TG> fp;
+      fp.grid = cg;
+      if (d == 1) {
+        fp.nx = nx;
+        fp.axis_stride = nx;
+        fp.n_outer = (long long)L * nz;
+        fp.outer_stride = nx * ny;
+      } else {
+        fp.nx = nx * ny;
+        fp.axis_stride = nx * ny;
+        fp.n_outer = L;
+        fp.outer_stride = G;
+      }
+      const int n1 = pl.n1[d], n2 = pl.n2[d];
+      if (n2 == 1) {
+        fp.m = n1;
+        fp.estride = 1;
+        fp.sub_stride = 0;
+        fp.n_sub = 1;
+        fp.twiddle = 0;
+        fp.ntot = n1;
+        fp.rad = radices_of(n1);
+        launch_fft_strided<TG>(fp, st);
+      } else {
+        fp.m = n1;
+        fp.estride = n2;
+        fp.sub_stride = 1;
+        fp.n_sub = n2;
+        fp.twiddle = 1;
+        fp.ntot = (long long)n1 * n2;
+        fp.rad = radices_of(n1);
+        launch_fft_strided<TG>(fp, st);
+        fp.m = n2;
+        fp.estride = 1;
+        fp.sub_stride = n2;
+        fp.n_sub = n1;
+        fp.twiddle = 0;
+        fp.rad = radices_of(n2);
+        launch_fft_strided<TG>(fp, st);
+      }
+    }
+
+    // interpolation, correction, real part into the images
+    double* dgl = gl.as<double>(2 * pl.gl_x.size());
+    BIPP_HIP(hipMemcpyAsync(dgl, pl.gl_x.data(), sizeof(double) * pl.gl_x.size(),
+                            hipMemcpyHostToDevice, st));
+    BIPP_HIP(hipMemcpyAsync(dgl + pl.gl_x.size(), pl.gl_w.data(), sizeof(double) * pl.gl_w.size(),
+                            hipMemcpyHostToDevice, st));
+    N3InterpParams<TG> ip;
+    ip.n_pix = n_pix;
+    ip.n_levels = L;
+    ip.w1 = pl.w;
+    ip.w2 = pl.w;
+    ip.beta1 = pl.beta;
+    ip.beta2 = pl.beta;
+    for (int d = 0; d < 3; ++d) {
+      ip.gam[d] = pl.gam[d];
+      ip.D[d] = pl.D[d];
+      ip.C[d] = pl.C[d];
+      ip.h1[d] = pl.h1[d];
+      ip.h2[d] = pl.h2[d];
+      ip.nf2[d] = pl.nf2[d];
+      ip.n1[d] = pl.n1[d];
+      ip.n2[d] = pl.n2[d];
+    }
+    ip.lmn = lmn64;
+    ip.gl_x = dgl;
+    ip.gl_w = dgl + pl.gl_x.size();
+    ip.nq = static_cast<int>(pl.gl_x.size());
+    ip.grid = cg;
+    ip.grid_stride = G;
+    ip.img = img;
+    ip.scale = 1.0;
+    ip.img_ld = n_pix;
+    launch_n3_interp<TG, TG>(ip, st);
+  }
+};
+
+}  // namespace bippamd
 
 // ---------------------------------------------------------------------------------------
 // Fused synthesis pipeline
@@ -375,11 +747,40 @@ struct BippSynthesisImpl {
   DevBuf n_eig, n_ant_used, pos, afrag, wts;
   // direct-sum buffers
   DevBuf wdense, vfull, vv, vv32, uvw;
+  // type-3 NUFFT state
+  double tol = 1e-5;
+  double pix_lo[3] = {0, 0, 0}, pix_hi[3] = {0, 0, 0};
+  N3Plan n3plan;
+  N3Runner<float> n3f;
+  N3Runner<double> n3d;
+  DevBuf ranges, dummy;
+  int last_nf2[3] = {0, 0, 0};
   // staging for host collects
   DevBuf st_s, st_w, st_xyz, st_wl;
   std::vector<double> last_d;
   std::vector<int32_t> last_nred;
   int last_count = 0;
+  // stage timings on the context stream: 0 eigensolve, 1 prep/weights, 2 image, 3 launches
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double stage_ms[4] = {0, 0, 0, 0};
+
+  void init_events() {
+    for (auto& e : ev) BIPP_HIP(hipEventCreate(&e));
+  }
+  ~BippSynthesisImpl() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void accumulate_times() {
+    float t = 0;
+    BIPP_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+    stage_ms[0] += t;
+    BIPP_HIP(hipEventElapsedTime(&t, ev[1], ev[2]));
+    stage_ms[1] += t;
+    BIPP_HIP(hipEventElapsedTime(&t, ev[2], ev[3]));
+    stage_ms[2] += t;
+    stage_ms[3] += 1;
+  }
 
   bool img_double() const { return compute == BIPP_COMPUTE_FP64; }
   int steer_mode() const {
@@ -393,8 +794,10 @@ struct BippSynthesisImpl {
                     long long stride_w, const double* xyz, long long ldxyz, long long stride_xyz) {
     ContextImpl& c = *ctx->impl;
     hipStream_t st = c.stream;
+    BIPP_HIP(hipEventRecord(ev[0], st));
     er.run(c, nS, n_ant, n_beam, wl, s, s_double, lds, stride_s, w, s_double, ldw, stride_w, xyz,
            ldxyz, stride_xyz, false, 1.0);
+    BIPP_HIP(hipEventRecord(ev[1], st));
     const double* d_int = static_cast<const double*>(intervals.p);
     if (imager == 0) {
       const int mode = steer_mode();
@@ -445,6 +848,7 @@ struct BippSynthesisImpl {
       // rows of V_full are all antennas; n_red then counts eigenvectors only
       pp.n_rows_const = er.general_w ? n_ant : 0;
       launch_steer_prep<R>(pp, st);
+      BIPP_HIP(hipEventRecord(ev[2], st));
       SteerParams sp;
       sp.n_samples = nS;
       sp.n_levels = n_levels;
@@ -466,11 +870,81 @@ struct BippSynthesisImpl {
       sp.img = img.p;
       sp.img_ld = n_pix;
       launch_steer(sp, mode, st);
+      BIPP_HIP(hipEventRecord(ev[3], st));
+    } else if (imager == 2) {
+      // virtual visibilities + type-3 NUFFT over all samples and levels of the batch
+      R* wd = wdense.as<R>((size_t)nS * n_levels * n_beam);
+      launch_level_weights<R>(nS, n_levels, filter, d_int, er.d_n_red(), er.d_d(), n_beam, n_beam,
+                              er.d_n_vis(), wd, (long long)n_levels * n_beam, st);
+      // positions / wavelength of the rows used (operand fragments not needed)
+      const int apad = ((n_ant + 15) / 16) * 16;
+      SteerPrepParams<R> pp{};
+      pp.n_samples = nS;
+      pp.n_levels = n_levels;
+      pp.filter = filter;
+      pp.intervals = d_int;
+      pp.n_red = er.d_n_red();
+      pp.idx = er.general_w ? nullptr : er.d_idx();
+      pp.idx_stride = n_beam;
+      pp.d = er.d_d();
+      pp.stride_d = n_beam;
+      pp.n_cols = n_beam;
+      pp.n_vis = er.d_n_vis();
+      pp.z = er.general_w ? er.d_vfull() : er.d_z();
+      pp.ldz = er.general_w ? n_ant : n_beam;
+      pp.stride_z = er.general_w ? (long long)n_ant * n_beam : (long long)n_beam * n_beam;
+      pp.xyz = xyz;
+      pp.ldxyz = ldxyz;
+      pp.stride_xyz = stride_xyz;
+      pp.wl = wl;
+      pp.n_eig = n_eig.as<int>(nS);
+      pp.n_ant = n_ant_used.as<int>(nS);
+      pp.pos_stride = 3LL * apad;
+      pp.pos = pos.as<double>((size_t)nS * pp.pos_stride);
+      pp.apad = apad;
+      pp.afrag = dummy.get(256);
+      pp.afrag_stride = 0;
+      pp.kchunks = 1;
+      pp.rtiles = 0;
+      pp.wts = dummy.get(256);
+      pp.w_stride = 0;
+      pp.epad = 0;
+      pp.mode = steer_mode();
+      pp.n_rows_const = er.general_w ? n_ant : 0;
+      launch_steer_prep<R>(pp, st);
+      // input-point extent from the per-sample position ranges
+      double* rg = ranges.as<double>(6 * (size_t)nS);
+      launch_pos_range(nS, pp.pos, pp.pos_stride, apad, pp.n_ant, 0, rg, st);
+      std::vector<double> h_rg(6 * (size_t)nS);
+      BIPP_HIP(hipMemcpyAsync(h_rg.data(), rg, sizeof(double) * h_rg.size(), hipMemcpyDeviceToHost, st));
+      BIPP_HIP(hipStreamSynchronize(st));
+      double X[3] = {0, 0, 0}, Cc[3] = {0, 0, 0}, S[3], D[3];
+      for (int k = 0; k < nS; ++k)
+        for (int d = 0; d < 3; ++d)
+          X[d] = std::max(X[d], 2 * 3.14159265358979323846 * (h_rg[6 * k + 3 + d] - h_rg[6 * k + d]));
+      for (int d = 0; d < 3; ++d) {
+        S[d] = 0.5 * (pix_hi[d] - pix_lo[d]);
+        D[d] = 0.5 * (pix_hi[d] + pix_lo[d]);
+      }
+      n3plan.make(tol, X, S, Cc, D, compute == BIPP_COMPUTE_FP64 ? 4096 : 4096);
+      for (int d = 0; d < 3; ++d) last_nf2[d] = n3plan.nf2[d];
+      BIPP_HIP(hipEventRecord(ev[2], st));
+      const int rows_max = er.general_w ? n_ant : n_beam;
+      if (compute == BIPP_COMPUTE_FP64)
+        n3d.run<R>(c, n3plan, nS, n_levels, n_beam, rows_max, pp.n_ant, pp.z, pp.stride_z, pp.ldz,
+                   wd, pp.pos, pp.pos_stride, apad, static_cast<const double*>(lmn64.p), n_pix,
+                   img.p, false);
+      else
+        n3f.run<R>(c, n3plan, nS, n_levels, n_beam, rows_max, pp.n_ant, pp.z, pp.stride_z, pp.ldz,
+                   wd, pp.pos, pp.pos_stride, apad, static_cast<const double*>(lmn64.p), n_pix,
+                   img.p, compute == BIPP_COMPUTE_FP32);
+      BIPP_HIP(hipEventRecord(ev[3], st));
     } else {
       // virtual visibilities + direct NUFT sum, per sample and level
       R* wd = wdense.as<R>((size_t)nS * n_levels * n_beam);
       launch_level_weights<R>(nS, n_levels, filter, d_int, er.d_n_red(), er.d_d(), n_beam, n_beam,
                               er.d_n_vis(), wd, (long long)n_levels * n_beam, st);
+      BIPP_HIP(hipEventRecord(ev[2], st));
       cplx<R>* vf;
       if (er.general_w) {
         vf = const_cast<cplx<R>*>(er.d_vfull());
@@ -519,8 +993,10 @@ struct BippSynthesisImpl {
           }
         }
       }
+      BIPP_HIP(hipEventRecord(ev[3], st));
     }
     er.check(c);
+    accumulate_times();
     // keep the eigenvalues of this call for inspection
     last_count = nS;
     last_d.resize((size_t)nS * n_beam);
@@ -582,9 +1058,7 @@ BippError bipp_ctx_create(BippProcessingUnit pu, BippContext* ctx) {
 
 BippError bipp_ctx_destroy(BippContext ctx) {
   return guard([&] {
-    if (!ctx) return;
-    (void)hipStreamSynchronize(ctx->impl->stream);
-    delete ctx;
+    if (ctx) ctx->release();
   });
 }
 
@@ -773,7 +1247,7 @@ BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager
     require(intervals && lmn, BIPP_INVALID_POINTER_ERROR, "null array");
     require(nLevels >= 1 && nLevels <= 8, BIPP_INVALID_PARAMETER_ERROR, "1..8 levels supported");
     require(nBeam >= 1 && nBeam <= 512, BIPP_INVALID_PARAMETER_ERROR, "1..512 beams supported");
-    require(imager == 0 || imager == 1, BIPP_INVALID_PARAMETER_ERROR, "unknown imager");
+    require(imager >= 0 && imager <= 2, BIPP_INVALID_PARAMETER_ERROR, "unknown imager");
     require(ldlmn >= nPixel, BIPP_INVALID_PARAMETER_ERROR, "ldlmn < nPixel");
     require(compute == BIPP_COMPUTE_FP64 || compute == BIPP_COMPUTE_FP32 ||
                 compute == BIPP_COMPUTE_MIXED,
@@ -788,6 +1262,7 @@ BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager
     h->n_pix = (int)nPixel;
     h->filter = filter;
     h->batched = batched != 0;
+    h->init_events();
     hipStream_t st = ctx->impl->stream;
     double* di = h->intervals.as<double>(2 * nLevels);
     BIPP_HIP(hipMemcpyAsync(di, intervals, sizeof(double) * 2 * nLevels, hipMemcpyHostToDevice, st));
@@ -798,6 +1273,11 @@ BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager
         l32[c3 * nPixel + q] = lmn[q + c3 * ldlmn];
         l64[c3 * nPixel + q] = static_cast<double>(lmn[q + c3 * ldlmn]);
       }
+    for (int c3 = 0; c3 < 3; ++c3) {
+      const auto mm = std::minmax_element(l64.begin() + c3 * nPixel, l64.begin() + (c3 + 1) * nPixel);
+      h->pix_lo[c3] = nPixel ? *mm.first : 0.0;
+      h->pix_hi[c3] = nPixel ? *mm.second : 0.0;
+    }
     double* d64 = h->lmn64.as<double>(l64.size());
     float* d32 = h->lmn32.as<float>(l32.size());
     BIPP_HIP(hipMemcpyAsync(d64, l64.data(), sizeof(double) * l64.size(), hipMemcpyHostToDevice, st));
@@ -806,6 +1286,7 @@ BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager
     void* im = h->img.get(isz * nLevels * nPixel);
     BIPP_HIP(hipMemsetAsync(im, 0, isz * nLevels * nPixel, st));
     BIPP_HIP(hipStreamSynchronize(st));
+    ctx->retain();
     *out = h.release();
   });
 }
@@ -813,8 +1294,10 @@ BippError bipp_synthesis_create(BippContext ctx, BippCompute compute, int imager
 BippError bipp_synthesis_destroy(BippSynthesis h) {
   return guard([&] {
     if (!h) return;
-    (void)hipStreamSynchronize(h->ctx->impl->stream);
+    BippContextImpl* c = h->ctx;
+    (void)hipStreamSynchronize(c->impl->stream);
     delete h;
+    c->release();
   });
 }
 
@@ -908,6 +1391,29 @@ BippError bipp_synthesis_reset(BippSynthesis h) {
     BIPP_HIP(hipMemsetAsync(h->img.p, 0, isz * h->n_levels * h->n_pix, st));
     BIPP_HIP(hipStreamSynchronize(st));
     h->n_collected = 0;
+  });
+}
+
+BippError bipp_synthesis_set_tolerance(BippSynthesis h, double tol) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(tol > 0 && tol < 1, BIPP_INVALID_PARAMETER_ERROR, "tolerance must be in (0, 1)");
+    h->tol = tol;
+  });
+}
+
+BippError bipp_synthesis_nufft_grid(BippSynthesis h, int32_t* nf) {
+  return guard([&] {
+    require(h && nf, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    for (int d = 0; d < 3; ++d) nf[d] = h->last_nf2[d];
+  });
+}
+
+BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    if (ms) std::copy(h->stage_ms, h->stage_ms + 4, ms);
+    if (reset) std::fill(h->stage_ms, h->stage_ms + 4, 0.0);
   });
 }
 

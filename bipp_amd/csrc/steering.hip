@@ -161,8 +161,19 @@ __global__ __launch_bounds__(256) void steer_prep_kernel(SteerPrepParams<R> p) {
     const long long tk = it >> 6;
     const int kc = static_cast<int>(tk % p.kchunks);
     const int rt = static_cast<int>(tk / p.kchunks);
-    const int row = rt * 16 + (lane & 15);
-    const int ec = row >> 1, comp = row & 1;
+    // Output rows of a 16-row tile hold (Re, Im) of 8 eigenvectors.  The pairing follows the
+    // accumulator layout of the MFMA used, so that each lane holds both parts of an
+    // eigenvector: f32 16x16 tiles put rows 4g..4g+3 in lane group g (pairs (2k, 2k+1));
+    // v_mfma_f64_16x16x4f64 puts rows g, g+4, g+8, g+12 in lane group g (pairs (r, r+4)).
+    const int rho = lane & 15;
+    int ec, comp;
+    if (p.mode == STEER_FP64) {
+      ec = rt * 8 + (rho & 3) + 4 * (rho >> 3);
+      comp = (rho >> 2) & 1;
+    } else {
+      ec = rt * 8 + (rho >> 1);
+      comp = rho & 1;
+    }
     const bool rvalid = ec < ne;
     const int e = rvalid ? elist[ec] : 0;
     float vals32[8];
@@ -458,9 +469,12 @@ __global__ __launch_bounds__(SteerCfg<MODE>::NT) void steer_kernel(SteerParams p
           Img w0[RTW], w1[RTW];
 #pragma unroll
           for (int r = 0; r < RTW; ++r) {
-            const int e0 = (my_rt0 + r) * 8 + 2 * (lane >> 4);
+            // accumulator registers (0,1) and (2,3) of this lane hold eigenvectors e0, e1
+            const int g = lane >> 4;
+            const int e0 = (my_rt0 + r) * 8 + (MODE == STEER_FP64 ? g : 2 * g);
+            const int e1 = e0 + (MODE == STEER_FP64 ? 4 : 1);
             w0[r] = r < nmy ? wts[(size_t)lv * p.epad + e0] : Img(0);
-            w1[r] = r < nmy ? wts[(size_t)lv * p.epad + e0 + 1] : Img(0);
+            w1[r] = r < nmy ? wts[(size_t)lv * p.epad + e1] : Img(0);
           }
 #pragma unroll
           for (int c = 0; c < CT; ++c) {

@@ -58,14 +58,17 @@ class Workload:
 
 def make_lmn_grid(grid_size: int, fov_rad: float) -> np.ndarray:
     """(grid_size^2, 3) lmn pixel grid, row-major over (m, l) as frame.make_grids
-    flattens it (python/bipp/frame.py:55-72)."""
+    flattens it (python/bipp/frame.py:55-72).  Values are rounded to float32, the
+    precision in which the reference hands pixel coordinates to synthesis
+    (include/bipp/image_prop.hpp:20), and returned as float64."""
     lim = math.sin(fov_rad / 2)
     offset = lim / (grid_size / 2) * 0.5
     s1 = np.linspace(-lim - offset, lim - offset, grid_size)
     s2 = np.linspace(-lim + offset, lim + offset, grid_size)
     l_grid, m_grid = np.meshgrid(s2, s1)
     n_grid = np.sqrt(1 - l_grid ** 2 - m_grid ** 2)
-    return np.stack((l_grid, m_grid, n_grid), axis=0).reshape(3, -1).T.copy()
+    lmn = np.stack((l_grid, m_grid, n_grid), axis=0).reshape(3, -1).T
+    return lmn.astype(np.float32).astype(np.float64)
 
 
 def disk_positions(rng: np.random.Generator, n: int, diameter_m: float) -> np.ndarray:

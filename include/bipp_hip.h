@@ -96,7 +96,9 @@ BippError bipp_eigh_gram_batched(BippContext ctx, size_t nSamples, size_t nAnten
  * facade of earlier bipp releases.
  *   compute: FP64 / FP32 / MIXED (include/bipp/enums.h).
  *   imager:  0 = steering contraction (MFMA), 1 = virtual visibilities + direct NUFT sum
- *            (src/host/kernels/nuft_sum.cpp:9-31).
+ *            (src/host/kernels/nuft_sum.cpp:9-31), 2 = virtual visibilities + type-3 NUFFT
+ *            over all samples of a collect (replaces neonufft PlanT3,
+ *            src/host/nufft.hpp:196-216, src/gpu/nufft.hpp:198-216).
  *   intervals: nLevels x 2 (lo, hi) host doubles, closed intervals on the eigenvalues
  *            rounded to float32 (python/bipp/filter.py:48-51); filter: BippFilter.
  *   lmn: nPixel x 3 host float32 (ldlmn), the ImageProp::pixel_lmn layout
@@ -134,6 +136,16 @@ BippError bipp_synthesis_reset(BippSynthesis h);
 /* Eigenvalues (float64, nBeam per sample, ascending, zero padded) and kept-beam counts of
  * the samples of the last collect call, host arrays. */
 BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, double* d, int32_t* nBeamReduced);
+/* GPU time per stage accumulated over collect calls, from HIP events on the context stream:
+ * ms[0] eigensolve (flag, Gram, solve), ms[1] level weights / operand preparation,
+ * ms[2] imaging kernels, ms[3] number of collect batches timed.  The reference's analogue is
+ * its wall-clock timer tree (src/rt_graph.hpp:94-161).  reset != 0 clears the counters. */
+BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset);
+/* NUFFT tolerance of imager 2 (NufftSynthesisOptions::tolerance,
+ * include/bipp/image_synthesis.hpp:61-65); default 1e-5. */
+BippError bipp_synthesis_set_tolerance(BippSynthesis h, double tol);
+/* Inner FFT grid (nf2 per axis) of the last NUFFT collect, for inspection. */
+BippError bipp_synthesis_nufft_grid(BippSynthesis h, int32_t* nf);
 
 /* ------------------------------------------------------------------------------------ *
  * Kernel-level entry points on device arrays (parity tests and custom pipelines).

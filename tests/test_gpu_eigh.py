@@ -10,6 +10,7 @@ import pytest
 
 import bipp_amd
 from bipp_amd import synthetic
+from bipp_amd.hip import DeviceArray
 from oracle import bipp_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -127,7 +128,6 @@ def test_not_positive_definite_gram_raises(ctx):
 def test_batched_ragged_device(ctx):
     """Ragged batch (10..64 active of 64 antennas) in one persistent launch through the
     device-array entry point, against the oracle per sample."""
-    import torch
     r = np.random.default_rng(11)
     n, ns = 64, 24
     p0 = synthetic.disk_positions(r, n, 300.0)
@@ -142,25 +142,22 @@ def test_batched_ragged_device(ctx):
         s[:, ~act] = 0
         samples.append(synthetic.Sample(0, k, wl, p0, s, act))
     s_cm, x_cm, wls = synthetic.stack_samples(samples, np.complex64)
-    dev = torch.device("cuda")
-    s_d = torch.from_numpy(s_cm.view(np.float32)).to(dev)
-    x_d = torch.from_numpy(x_cm).to(dev)
-    wl_d = torch.from_numpy(wls).to(dev)
-    d_d = torch.zeros((ns, n), dtype=torch.float64, device=dev)
-    v_d = torch.zeros((ns, n, n, 2), dtype=torch.float64, device=dev)
-    nr_d = torch.zeros(ns, dtype=torch.int32, device=dev)
-    sc_d = torch.zeros(ns, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
+    s_d = DeviceArray.from_numpy(s_cm)
+    x_d = DeviceArray.from_numpy(x_cm)
+    wl_d = DeviceArray.from_numpy(wls)
+    d_d = DeviceArray.zeros((ns, n), np.float64)
+    v_d = DeviceArray.zeros((ns, n, n), np.complex128)
+    nr_d = DeviceArray.zeros(ns, np.int32)
+    sc_d = DeviceArray.zeros(ns, np.float64)
     print("batched ragged eigensolve", flush=True)
-    bipp_amd.call("bipp_eigh_gram_batched", ctx.handle, ns, n, n, 0, 1, 1, wl_d.data_ptr(),
-                  s_d.data_ptr(), n, n * n, None, 0, 0, x_d.data_ptr(), n, 3 * n,
-                  d_d.data_ptr(), n, v_d.data_ptr(), n, n * n, nr_d.data_ptr(), sc_d.data_ptr())
+    bipp_amd.call("bipp_eigh_gram_batched", ctx.handle, ns, n, n, 0, 1, 1, wl_d.ptr, s_d.ptr, n,
+                  n * n, None, 0, 0, x_d.ptr, n, 3 * n, d_d.ptr, n, v_d.ptr, n, n * n, nr_d.ptr,
+                  sc_d.ptr)
     ctx.synchronize()
-    d = d_d.cpu().numpy()
-    vv = v_d.cpu().numpy()
-    v = (vv[..., 0] + 1j * vv[..., 1]).transpose(0, 2, 1)  # column-major per sample
-    nr = nr_d.cpu().numpy()
-    sc = sc_d.cpu().numpy()
+    d = d_d.numpy()
+    v = v_d.numpy().transpose(0, 2, 1)  # column-major per sample
+    nr = nr_d.numpy()
+    sc = sc_d.numpy()
     for k, smp in enumerate(samples):
         vo, do, nro, sco = O.eigh_gram(smp.wl, smp.s.astype(np.complex64), None, smp.xyz)
         assert nr[k] == nro

@@ -11,6 +11,7 @@ import pytest
 
 import bipp_amd
 from bipp_amd import synthetic
+from bipp_amd.hip import DeviceArray
 from oracle import bipp_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -33,10 +34,12 @@ def cfg1_golden(golden):
                 img_st=z["img_steering"], img_ds=z["img_direct"], d=z["d"])
 
 
-# (compute, imager, relative L2 tolerance against the float64 golden image)
+# (compute, imager, relative L2 tolerance against the float64 golden image); images are
+# returned as float32 like the reference's (src/image_synthesis.cpp:124-127), which bounds
+# the float64 forms at a few 1e-8
 CASES = [
-    ("fp64", "steering", 1e-9),
-    ("fp64", "direct", 1e-9),
+    ("fp64", "steering", 1e-7),
+    ("fp64", "direct", 1e-7),
     ("mixed", "steering", 1e-4),
     ("mixed", "direct", 1e-4),
     ("fp32", "steering", 5e-2),
@@ -81,7 +84,7 @@ def test_collect_is_additive_and_reset(ctx, cfg1_golden):
     assert syn.num_collected == len(g["wl"])
     img = syn.images()
     for lvl in range(img.shape[0]):
-        assert O.rel_l2(img[lvl], g["img_st"][lvl]) < 1e-9
+        assert O.rel_l2(img[lvl], g["img_st"][lvl]) < 1e-7
     syn.reset()
     assert syn.num_collected == 0 and np.all(syn.get(0, normalize=False) == 0)
 
@@ -118,6 +121,10 @@ def test_cfg2_full_size_subgrid(ctx):
     """Full-size 256-antenna samples (config 2 geometry) on a 64x64 subgrid: split-bf16
     steering and the float32 form against the float64 oracle."""
     wl_ = synthetic.make_workload(2, n_timesteps=2, grid=64)
+    # the GPU receives complex64 visibilities; the oracle sees the same rounded values
+    # (the noise-cluster level is sensitive to that rounding at the 1e-6 level)
+    for smp in wl_.samples:
+        smp.s = smp.s.astype(np.complex64).astype(np.complex128)
     evs = [O.eigh_gram(s.wl, s.s, None, s.xyz)[1] for s in wl_.samples]
     intervals = O.infer_intervals(4, 1.0, "log", 0.0, float("inf"), evs)
     ref, _ = O.synthesize(wl_.samples, intervals, wl_.lmn)
@@ -134,14 +141,7 @@ def test_cfg2_full_size_subgrid(ctx):
 
 
 # ---------------------------------------------------------------- kernel entry points
-@pytest.fixture(scope="module")
-def torch_dev():
-    import torch
-    return torch, torch.device("cuda")
-
-
-def test_nuft_sum_kernel(ctx, torch_dev):
-    torch, dev = torch_dev
+def test_nuft_sum_kernel(ctx):
     r = np.random.default_rng(3)
     n_in, n_out = 1000, 777
     uvw = r.normal(size=(n_in, 3)) * 300
@@ -149,67 +149,119 @@ def test_nuft_sum_kernel(ctx, torch_dev):
     lmn = synthetic.make_lmn_grid(28, math.radians(8))[:n_out]
     ref = O.nuft_sum(1.0, c, uvw, lmn)
     for prec, tol in (("double", 1e-11), ("single", 1e-3)):
-        dt = torch.float64 if prec == "double" else torch.float32
-        cd = np.complex128 if prec == "double" else np.complex64
-        cin = torch.from_numpy(c.astype(cd).view(np.float64 if prec == "double" else np.float32)).to(dev)
-        u = [torch.tensor(uvw[:, i], dtype=dt, device=dev) for i in range(3)]
-        x = [torch.tensor(lmn[:, i], dtype=dt, device=dev) for i in range(3)]
-        out = torch.zeros(n_out, dtype=dt, device=dev)
-        torch.cuda.synchronize()
-        bipp_amd.call("bipp_nuft_sum", ctx.handle, bipp_amd.PRECISION[prec], 1.0, n_in,
-                      cin.data_ptr(), *[t.data_ptr() for t in u], n_out,
-                      *[t.data_ptr() for t in x], out.data_ptr())
+        rd, cd = (np.float64, np.complex128) if prec == "double" else (np.float32, np.complex64)
+        cin = DeviceArray.from_numpy(c.astype(cd))
+        u = [DeviceArray.from_numpy(uvw[:, i].astype(rd)) for i in range(3)]
+        x = [DeviceArray.from_numpy(lmn[:, i].astype(rd)) for i in range(3)]
+        out = DeviceArray.zeros(n_out, rd)
+        bipp_amd.call("bipp_nuft_sum", ctx.handle, bipp_amd.PRECISION[prec], 1.0, n_in, cin.ptr,
+                      *[t.ptr for t in u], n_out, *[t.ptr for t in x], out.ptr)
         ctx.synchronize()
-        err = O.rel_l2(out.cpu().numpy(), ref)
+        err = O.rel_l2(out.numpy(), ref)
         print(f"nuft_sum {prec}: rel {err:.2e}")
         assert err < tol
-    cin = torch.from_numpy(c.astype(np.complex64).view(np.float32)).to(dev)
-    u = [torch.tensor(uvw[:, i], dtype=torch.float64, device=dev) for i in range(3)]
-    x = [torch.tensor(lmn[:, i], dtype=torch.float64, device=dev) for i in range(3)]
-    out = torch.zeros(n_out, dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    bipp_amd.call("bipp_nuft_sum_mixed", ctx.handle, 1.0, n_in, cin.data_ptr(),
-                  *[t.data_ptr() for t in u], n_out, *[t.data_ptr() for t in x], out.data_ptr())
+    cin = DeviceArray.from_numpy(c.astype(np.complex64))
+    u = [DeviceArray.from_numpy(uvw[:, i].copy()) for i in range(3)]
+    x = [DeviceArray.from_numpy(lmn[:, i].copy()) for i in range(3)]
+    out = DeviceArray.zeros(n_out, np.float32)
+    bipp_amd.call("bipp_nuft_sum_mixed", ctx.handle, 1.0, n_in, cin.ptr, *[t.ptr for t in u],
+                  n_out, *[t.ptr for t in x], out.ptr)
     ctx.synchronize()
     ref32 = O.nuft_sum(1.0, c.astype(np.complex64).astype(np.complex128), uvw, lmn)
-    err = O.rel_l2(out.cpu().numpy(), ref32)
+    err = O.rel_l2(out.numpy(), ref32)
     print(f"nuft_sum mixed: rel {err:.2e}")
     assert err < 1e-5
 
 
-def test_virtual_vis_kernel(ctx, torch_dev):
-    torch, dev = torch_dev
+def test_virtual_vis_kernel(ctx):
     r = np.random.default_rng(4)
     ns, n, k = 3, 29, 21
     v = r.normal(size=(ns, n, k)) + 1j * r.normal(size=(ns, n, k))
     dw = r.normal(size=(ns, k))
-    v_cm = np.ascontiguousarray(np.swapaxes(v, 1, 2))
-    vt = torch.from_numpy(v_cm.view(np.float64)).to(dev)
-    dt = torch.from_numpy(dw).to(dev)
-    out = torch.zeros((ns, n * n * 2), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    bipp_amd.call("bipp_virtual_vis", ctx.handle, 1, ns, n, k, vt.data_ptr(), n, n * k,
-                  dt.data_ptr(), k, out.data_ptr(), n * n)
+    vt = DeviceArray.from_numpy(np.ascontiguousarray(np.swapaxes(v, 1, 2)))
+    dt = DeviceArray.from_numpy(dw)
+    out = DeviceArray.zeros((ns, n * n), np.complex128)
+    bipp_amd.call("bipp_virtual_vis", ctx.handle, 1, ns, n, k, vt.ptr, n, n * k, dt.ptr, k,
+                  out.ptr, n * n)
     ctx.synchronize()
-    o = out.cpu().numpy().view(np.complex128)
+    o = out.numpy()
     for s in range(ns):
         assert np.allclose(o[s], O.virtual_vis(1.0, dw[s], v[s]), rtol=1e-12, atol=1e-12)
 
 
-def test_gram_kernel(ctx, torch_dev):
-    torch, dev = torch_dev
+def test_gram_kernel(ctx):
     r = np.random.default_rng(5)
     ns, n = 2, 33
     xyz = r.normal(size=(ns, n, 3)) * 40
     wl = np.array([1.7, 2.3])
-    xt = torch.from_numpy(np.ascontiguousarray(np.swapaxes(xyz, 1, 2))).to(dev)
-    wt = torch.from_numpy(wl).to(dev)
-    g = torch.zeros((ns, n * n * 2), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    bipp_amd.call("bipp_gram", ctx.handle, 1, ns, n, wt.data_ptr(), xt.data_ptr(), n, 3 * n,
-                  4 * np.pi, g.data_ptr(), n, n * n)
+    xt = DeviceArray.from_numpy(np.ascontiguousarray(np.swapaxes(xyz, 1, 2)))
+    wt = DeviceArray.from_numpy(wl)
+    g = DeviceArray.zeros((ns, n * n), np.complex128)
+    bipp_amd.call("bipp_gram", ctx.handle, 1, ns, n, wt.ptr, xt.ptr, n, 3 * n, 4 * np.pi, g.ptr,
+                  n, n * n)
     ctx.synchronize()
-    gg = g.cpu().numpy().view(np.complex128)
+    gg = g.numpy()
     for s in range(ns):
         ref = O.gram_matrix(np.eye(n), xyz[s], wl[s], prefactor=4 * np.pi)
         assert np.allclose(gg[s].reshape(n, n, order="F"), ref, rtol=1e-12, atol=1e-13)
+
+
+# ---------------------------------------------------------------- type-3 NUFFT imager
+@pytest.mark.parametrize("compute,tol,bound", [("fp64", 1e-9, 1e-7), ("mixed", 1e-5, 1e-4),
+                                               ("fp32", 1e-5, 5e-2)])
+def test_nufft_cfg1_golden(ctx, cfg1_golden, compute, tol, bound):
+    """Virtual visibilities + type-3 NUFFT of all samples, against the float64 golden."""
+    g = cfg1_golden
+    print(f"nufft cfg1 compute={compute} tol={tol}", flush=True)
+    syn = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute=compute,
+                                  imager="nufft", tolerance=tol)
+    syn.collect(g["wl"], g["s"], g["xyz"])
+    img = syn.images()
+    print("  grid", syn.nufft_grid())
+    for lvl in range(img.shape[0]):
+        err = O.rel_l2(img[lvl], g["img_st"][lvl])
+        print(f"  level {lvl}: rel L2 {err:.3e}")
+        assert err <= bound
+
+
+def test_nufft_lofar_general_w(ctx, golden):
+    z = golden("lofar_input.npz")
+    e = golden("lofar_expected.npz")
+    xyz = np.stack([_lofar_positions(z, k) for k in range(z["s"].shape[0])])
+    wl = np.full(z["s"].shape[0], float(z["wl"]))
+    syn = bipp_amd.NufftSynthesis(ctx, 55, 55, z["intervals"], e["lmn"], compute="fp64",
+                                  imager="nufft", tolerance=1e-9)
+    syn.collect(wl, z["s"], xyz, z["w"])
+    img = syn.images()
+    for lvl in range(img.shape[0]):
+        err = O.rel_l2(img[lvl], e["img_f64"][lvl])
+        print(f"  level {lvl}: rel L2 {err:.3e}")
+        assert err <= 1e-5
+
+
+def test_nufft_cfg4_geometry_subgrid(ctx):
+    """Config-4 geometry (256 antennas, 10 km, 2048^2 pixels over 5 deg): the NUFFT image on
+    the full grid, checked on a 64x64 subgrid of pixels against the float64 steering oracle."""
+    wl_ = synthetic.make_workload(4, n_timesteps=3)
+    for smp in wl_.samples:
+        smp.s = smp.s.astype(np.complex64).astype(np.complex128)
+    evs = [O.eigh_gram(s.wl, s.s, None, s.xyz)[1] for s in wl_.samples]
+    intervals = O.infer_intervals(4, 1.0, "log", 0.0, float("inf"), evs)
+    sub = np.arange(0, 2048, 32)
+    qidx = (sub[:, None] * 2048 + sub[None, :]).ravel()
+    ref, _ = O.synthesize(wl_.samples, intervals, wl_.lmn[qidx])
+    s = np.stack([x.s for x in wl_.samples]).astype(np.complex64)
+    xyz = np.stack([x.xyz for x in wl_.samples])
+    wl = np.array([x.wl for x in wl_.samples])
+    for compute, tol, bound in (("mixed", 1e-5, 1e-4), ("fp64", 1e-8, 1e-6)):
+        print(f"cfg4 nufft compute={compute}", flush=True)
+        syn = bipp_amd.NufftSynthesis(ctx, 256, 256, intervals, wl_.lmn, compute=compute,
+                                      imager="nufft", tolerance=tol)
+        syn.collect(wl, s, xyz)
+        img = syn.images()[:, qidx]
+        print("  grid", syn.nufft_grid(), syn.timings())
+        for lvl in range(4):
+            err = O.rel_l2(img[lvl], ref[lvl])
+            print(f"  level {lvl}: rel L2 {err:.3e}")
+            assert err <= bound
+        del syn
