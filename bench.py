@@ -1,15 +1,15 @@
 """Benchmark: (timestep, channel) samples imaged per second on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu]
 
 A step is one pass of the fused Bluebild chain over the configuration's whole batch of
 synthetic samples (flag -> Gram -> generalized eigensolve -> energy levels -> image),
 with the samples already resident in HBM.  `value` counts samples per second of the
-mixed/batched form (float64 eigensolve and phase reduction, split-bf16 MFMA contraction),
-valid only when every level image is within 1e-4 relative L2 of the float64 GPU run of
-the same inputs and the level masks are identical.  The plain float32 per-sample form is
-timed beside it, and the CPU restatement of the reference path is timed on the host.
-Prints one JSON line.
+mixed/batched form (float64 eigensolve and phase reduction, split-bf16 MFMA contraction or
+float32-grid type-3 NUFFT), and is reported only when every level image is within 1e-4
+relative L2 of a float64 GPU run of the same inputs and the level masks are identical.
+The plain float32 per-sample form is timed beside it, and the CPU restatement of the
+reference path is timed on the host.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -30,8 +30,8 @@ from bipp_amd import hip, synthetic  # noqa: E402
 from bipp_amd.hip import DeviceArray  # noqa: E402
 
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
-PEAK_FP32_TFLOPS = 157.3
-PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+IMAGER = {1: "steering", 2: "steering", 3: "steering", 4: "nufft", 5: "nufft"}
 
 
 def log(msg: str) -> None:
@@ -58,23 +58,24 @@ def level_intervals(ctx, dev, n_s, n, n_levels):
     return bipp_amd.infer_intervals(n_levels, 1.0, "log", 0.0, float("inf"), d), d
 
 
-def eig_count(d, intervals):
-    """Eigenvectors with a nonzero weight in some level (rows of the contraction)."""
+def masks(d, intervals):
     df = d.astype(np.float32)
-    m = np.zeros(df.shape, bool)
-    for lo, hi in intervals:
-        m |= (df >= np.float32(lo)) & (df <= np.float32(hi)) & (df != 0)
-    return m.sum(axis=1)
+    return np.stack([(df >= np.float32(lo)) & (df <= np.float32(hi)) & (df != 0)
+                     for lo, hi in intervals])
 
 
-def run_form(ctx, wl_, dev, intervals, compute, batched, steps, warmup):
+def run_form(ctx, n, dev, n_s, lmn, intervals, compute, imager, batched, steps, warmup,
+             tol=None, first=0):
+    """Time `steps` passes over samples [first, first + n_s) of the device arrays."""
     s_d, x_d, wl_d = dev
-    n_s, n = wl_.n_samples, wl_.n_antenna
-    syn = bipp_amd.NufftSynthesis(ctx, n, n, intervals, wl_.lmn, compute=compute,
-                                  imager="steering", batched=batched)
+    syn = bipp_amd.NufftSynthesis(ctx, n, n, intervals, lmn, compute=compute, imager=imager,
+                                  batched=batched, tolerance=tol)
+    s_ptr = s_d.ptr + first * n * n * 8
+    x_ptr = x_d.ptr + first * 3 * n * 8
+    w_ptr = wl_d.ptr + first * 8
     for _ in range(warmup):
         syn.reset()
-        syn.collect_device(n_s, wl_d.ptr, s_d.ptr, False, x_d.ptr)
+        syn.collect_device(n_s, w_ptr, s_ptr, False, x_ptr)
     hip.synchronize()
     syn.timings(reset=True)
     times = []
@@ -82,21 +83,24 @@ def run_form(ctx, wl_, dev, intervals, compute, batched, steps, warmup):
         syn.reset()
         hip.synchronize()
         t0 = time.perf_counter()
-        syn.collect_device(n_s, wl_d.ptr, s_d.ptr, False, x_d.ptr)
+        syn.collect_device(n_s, w_ptr, s_ptr, False, x_ptr)
         hip.synchronize()
         times.append(time.perf_counter() - t0)
-    tm = syn.timings()
-    return syn, float(np.mean(times)), tm
+    return syn, float(np.mean(times)), syn.timings()
 
 
-def cpu_baseline(wl_, intervals, n_samples=2, grid=96):
+def rel_errs(a, b):
+    return [float(np.linalg.norm(a[l] - b[l]) / np.linalg.norm(b[l])) for l in range(b.shape[0])]
+
+
+def cpu_baseline(wl_, intervals, n_samples=4, grid=192):
     """CPU restatement of the reference path (float32 chegv + closed-form image, OpenMP),
     on `n_samples` samples and a grid x grid pixel subset, scaled to the full grid."""
     from oracle import cpu as ocpu
 
     L = ocpu.lib()
     cores = L.bippc_max_threads()
-    lm_full = wl_.lmn
+    n_full = wl_.lmn.shape[0]
     sub = synthetic.make_lmn_grid(grid, math.radians(wl_.fov_deg))
     t_eig = t_img = 0.0
     for smp in wl_.samples[:n_samples]:
@@ -112,14 +116,16 @@ def cpu_baseline(wl_, intervals, n_samples=2, grid=96):
         t0 = time.perf_counter()
         ocpu.steering_image(v[:, keep], w[:, keep], smp.xyz / smp.wl, sub, img)
         t_img += time.perf_counter() - t0
-    per_sample = (t_eig + t_img * lm_full.shape[0] / sub.shape[0]) / n_samples
+    scale = n_full / sub.shape[0]
+    per_sample = (t_eig + t_img * scale) / n_samples
     return {"value": 1.0 / per_sample, "unit": "samples/s", "cores": int(cores), "kind": "port",
             "sample": (f"{n_samples} samples: float32 chegv eigensolve (scipy OpenBLAS, {cores} "
-                       f"threads) + closed-form image on a {grid}x{grid} pixel subset, image time "
-                       f"scaled by {lm_full.shape[0] // sub.shape[0]}x to the full grid; the "
-                       f"reference's neonufft CPU path cannot be built offline"),
+                       f"threads) + closed-form image (OpenMP, {cores} threads) on a {grid}x{grid} "
+                       f"pixel subset, image time scaled by {scale:.1f}x to the full "
+                       f"{int(math.sqrt(n_full))}^2 grid; the reference's neonufft CPU path "
+                       f"cannot be built offline"),
             "eig_s_per_sample": t_eig / n_samples,
-            "image_s_per_sample_full_grid": t_img * lm_full.shape[0] / sub.shape[0] / n_samples}
+            "image_s_per_sample_full_grid": t_img * scale / n_samples}
 
 
 def main() -> None:
@@ -127,8 +133,11 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--timesteps", type=int, default=None)
+    ap.add_argument("--imager", choices=["steering", "nufft"], default=None)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--plain-samples", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
@@ -136,47 +145,90 @@ def main() -> None:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the sample-parallel "
               f"multi-GPU form is described in DESIGN.md and not built", file=sys.stderr)
         sys.exit(2)
+    imager = a.imager or IMAGER[a.config]
 
     t_setup = time.perf_counter()
     wl_ = synthetic.make_workload(a.config, n_timesteps=a.timesteps)
     n_s, n, P = wl_.n_samples, wl_.n_antenna, wl_.lmn.shape[0]
-    log(f"config {a.config}: {n_s} samples, {n} antennas, {P} pixels, {wl_.n_levels} levels "
-        f"(setup {time.perf_counter() - t_setup:.1f}s)")
+    log(f"config {a.config}: {n_s} samples, {n} antennas, {P} pixels, {wl_.n_levels} levels, "
+        f"imager {imager} (setup {time.perf_counter() - t_setup:.1f}s)")
     ctx = bipp_amd.Context("GPU")
     dev = upload(wl_)
     intervals, d_all = level_intervals(ctx, dev, n_s, n, wl_.n_levels)
-    k_eig = eig_count(d_all, intervals)
-    flops = float(np.sum(8.0 * n * k_eig * P))   # complex GEMM V^T E per sample: 8 N K P
+    k_eig = masks(d_all, intervals).any(axis=0).sum(axis=1)
 
     # mixed / batched form (the measured value)
-    syn_m, t_m, tm_m = run_form(ctx, wl_, dev, intervals, "mixed", True, a.steps, a.warmup)
+    syn_m, t_m, tm_m = run_form(ctx, n, dev, n_s, wl_.lmn, intervals, "mixed", imager, True,
+                                a.steps, a.warmup, a.tol)
     img_m = syn_m.images()
     d_m, _ = syn_m.last_eigenvalues(n_s)
-    log(f"mixed batched: {t_m * 1e3:.1f} ms/step  stages {tm_m}")
+    nb = max(tm_m["batches"], 1)
+    stages = {k: v / nb for k, v in tm_m.items() if k != "batches"}
+    log(f"mixed batched: {t_m * 1e3:.1f} ms/step  stages {stages}")
 
-    # float64 GPU oracle of the same inputs
-    syn_o, t_o, _ = run_form(ctx, wl_, dev, intervals, "fp64", True, 1, 0)
+    # float64 GPU oracle of the same inputs: the exact steering form (full grid for the
+    # steering imager, a 64x64 pixel subgrid for the NUFFT imager)
+    if imager == "steering":
+        qidx = np.arange(P)
+    else:
+        g = wl_.grid
+        sub = np.arange(0, g, max(1, g // 64))
+        qidx = (sub[:, None] * g + sub[None, :]).ravel()
+    syn_o, t_o, _ = run_form(ctx, n, dev, n_s, wl_.lmn[qidx], intervals, "fp64", "steering",
+                             True, 1, 0)
     img_o = syn_o.images()
     d_o, _ = syn_o.last_eigenvalues(n_s)
-    errs_m = [float(np.linalg.norm(img_m[l] - img_o[l]) / np.linalg.norm(img_o[l]))
-              for l in range(len(intervals))]
-    masks_equal = bool(np.array_equal(eig_count_mask(d_m, intervals), eig_count_mask(d_o, intervals)))
-    log(f"fp64 oracle run {t_o * 1e3:.1f} ms; mixed rel L2 per level {errs_m}; masks equal {masks_equal}")
+    errs_m = rel_errs(img_m[:, qidx], img_o)
+    masks_equal = bool(np.array_equal(masks(d_m, intervals), masks(d_o, intervals)))
+    log(f"fp64 oracle ({len(qidx)} pixels) {t_o * 1e3:.1f} ms; mixed rel L2 per level {errs_m}; "
+        f"masks equal {masks_equal}")
+    del syn_o
 
     plain = None
     if not a.no_plain:
-        syn_p, t_p, tm_p = run_form(ctx, wl_, dev, intervals, "fp32", False, 1, 1)
-        img_p = syn_p.images()
-        errs_p = [float(np.linalg.norm(img_p[l] - img_o[l]) / np.linalg.norm(img_o[l]))
-                  for l in range(len(intervals))]
-        plain = {"samples_per_s": n_s / t_p, "ms_per_step": t_p * 1e3, "max_rel_err": max(errs_p),
-                 "form": "float32 everywhere, one launch sequence per sample"}
-        log(f"plain fp32 per-sample: {t_p * 1e3:.1f} ms/step, rel err {errs_p}")
+        n_p = n_s if imager == "steering" else min(n_s, a.plain_samples)
+        syn_p, t_p, _ = run_form(ctx, n, dev, n_p, wl_.lmn, intervals, "fp32", imager, False, 1,
+                                 1 if imager == "steering" else 0, a.tol)
+        img_p = syn_p.images()[:, qidx]
+        del syn_p
+        if n_p == n_s:
+            ref_p = img_o
+        else:
+            syn_q, _, _ = run_form(ctx, n, dev, n_p, wl_.lmn[qidx], intervals, "fp64", "steering",
+                                   True, 1, 0)
+            ref_p = syn_q.images()
+            del syn_q
+        errs_p = rel_errs(img_p, ref_p)
+        plain = {"samples_per_s": n_p / t_p, "ms_per_sample": t_p * 1e3 / n_p,
+                 "samples_timed": n_p, "max_rel_err": max(errs_p), "per_level": errs_p,
+                 "form": "float32 everywhere (phases included), one launch sequence per sample"}
+        log(f"plain fp32 per-sample: {t_p * 1e3:.1f} ms for {n_p} samples, rel err {errs_p}")
 
     ok = max(errs_m) <= 1e-4 and masks_equal
     value = n_s / t_m if ok else 0.0
-    kern_ms = tm_m["image_ms"] / max(tm_m["batches"], 1)
-    achieved = flops / (kern_ms * 1e-3) / 1e12
+
+    if imager == "steering":
+        flops = float(np.sum(8.0 * n * k_eig * P))   # complex GEMM V^T E: 8 N K P per sample
+        kern_ms = stages["image_ms"]
+        achieved = flops / (kern_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                    "kernel": "steer_kernel<BF16X3>", "kernel_ms": kern_ms,
+                    "algorithmic_flops_per_launch": flops,
+                    "mfma_flops_executed_per_launch": 3 * flops}
+    else:
+        nf = syn_m.nufft_grid()
+        cells = float(np.prod(nf))
+        L = len(intervals)
+        n_pass = 1 + sum(1 if x <= 128 else 2 for x in nf[1:])
+        fft_bytes = cells * L * 8.0 * 2 * n_pass        # complex64 read + write per pass
+        kern_ms = stages["nufft_fft_ms"]
+        achieved = fft_bytes / (kern_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                    "kernel": "deconvolution + Stockham FFT passes (fft_x_kernel, fft_strided_kernel)",
+                    "kernel_ms": kern_ms, "grid": list(nf), "passes": n_pass,
+                    "algorithmic_bytes_per_launch": fft_bytes}
     cpu = None
     if not a.no_cpu:
         try:
@@ -195,30 +247,21 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3-mfma/fp32-acc, fp64 eigensolve",
+        "dtype": ("bf16x3-mfma/fp32-acc, fp64 eigensolve" if imager == "steering"
+                  else "fp32 nufft grid, fp64 eigensolve and phases"),
         "data": "synthetic (seeded disk array, point-source sky, noise 0.1)",
         "config": {"workload": wl_.name, "samples": n_s, "antennas": n, "pixels": P,
-                   "levels": len(intervals), "imager": "steering", "batch": n_s},
+                   "levels": len(intervals), "imager": imager, "batch": n_s,
+                   "nufft_tolerance": a.tol if imager == "nufft" else None},
         "accuracy": {"max_rel_l2_vs_fp64": max(errs_m), "per_level": errs_m,
-                     "level_masks_identical": masks_equal},
+                     "pixels_checked": int(len(qidx)), "level_masks_identical": masks_equal},
         "plain_fp32": plain,
         "speedup_vs_plain": (value / plain["samples_per_s"]) if (plain and value) else None,
-        "stages_ms_per_step": {k: v / max(tm_m["batches"], 1) for k, v in tm_m.items()
-                               if k != "batches"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-                     "kernel": "steer_kernel<BF16X3>",
-                     "algorithmic_flops_per_launch": flops,
-                     "mfma_flops_executed_per_launch": 3 * flops},
+        "stages_ms_per_step": stages,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-
-
-def eig_count_mask(d, intervals):
-    df = d.astype(np.float32)
-    return np.stack([(df >= np.float32(lo)) & (df <= np.float32(hi)) & (df != 0)
-                     for lo, hi in intervals])
 
 
 if __name__ == "__main__":

@@ -326,10 +326,12 @@ class NufftSynthesis:
 
     def timings(self, reset: bool = False) -> Dict[str, float]:
         """GPU milliseconds per stage accumulated over collect calls (HIP events)."""
-        ms = np.zeros(4)
+        ms = np.zeros(9)
         check(lib().bipp_synthesis_timings(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
                                            1 if reset else 0))
-        return {"eigensolve_ms": ms[0], "prep_ms": ms[1], "image_ms": ms[2], "batches": ms[3]}
+        keys = ("eigensolve_ms", "prep_ms", "image_ms", "batches", "nufft_vv_ms",
+                "nufft_sort_ms", "nufft_spread_ms", "nufft_fft_ms", "nufft_interp_ms")
+        return {k: float(v) for k, v in zip(keys, ms)}
 
     def last_eigenvalues(self, n_samples: int):
         d = np.zeros((n_samples, self.n_beam), dtype=np.float64)

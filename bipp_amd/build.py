@@ -33,7 +33,10 @@ def hipcc() -> str:
 def flags(debug: bool = False):
     f = ["-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + os.path.join(ROOT, "include"),
          "-I" + CSRC, "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # device buffers come from hipMalloc (coarse-grained): floating-point atomics can
+         # use the hardware add instead of a compare-and-swap loop
+         "-munsafe-fp-atomics"]
     f += ["-O1", "-g"] if debug else ["-O3"]
     return f
 

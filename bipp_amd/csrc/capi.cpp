@@ -470,7 +470,9 @@ struct N3Plan {
   double beta = 13.8;
   int nf1[3], nf2[3], n1[3], n2[3];
   double h1[3], h2[3], gam[3], C[3], D[3];
-  int bin[3] = {8, 8, 8}, nbins[3], off[3];
+  int bin[3] = {8, 8, 8}, nbins[3], nback[3], off[3];
+  // FFT windows: spread input l in [-nf1/2, nf1/2) and the cells the pixels interpolate from
+  N3Window in_w[3], out_w[3];
   long long grid_cells = 0;
   std::vector<double> gl_x, gl_w;
   std::vector<double> deconv[3];
@@ -502,9 +504,24 @@ struct N3Plan {
       } else {
         split_axis(nf2[d], n1[d], n2[d]);
       }
-      off[d] = nf1[d] / 2 + w;
-      nbins[d] = (nf1[d] + 2 * w) / bin[d] + 1;
+      // spread blocks over l + nf1/2 in [0, nf1); a point's support [l0, l0 + w) starts in
+      // its bin and reaches at most nback blocks further
+      off[d] = nf1[d] / 2;
+      nbins[d] = (nf1[d] + bin[d] - 1) / bin[d];
+      nback[d] = 1 + (w - 2) / bin[d];
       grid_cells *= nf2[d];
+      in_w[d] = N3Window{nf2[d] - nf1[d] / 2, nf1[d]};
+      // pixels sit at |a| <= h1 gam S / h2 grid cells and read cells ceil(a - w/2) + [0, w);
+      // one cell of margin on each side
+      const double amax = h1[d] * gam[d] * S / h2[d];
+      const long long klo = static_cast<long long>(std::floor(-amax - 0.5 * w)) - 1;
+      const long long khi = static_cast<long long>(std::ceil(amax - 0.5 * w)) + w;
+      const long long cnt = khi - klo + 1;
+      if (cnt >= nf2[d])
+        out_w[d] = N3Window{0, nf2[d]};
+      else
+        out_w[d] = N3Window{static_cast<int>(((klo % nf2[d]) + nf2[d]) % nf2[d]),
+                            static_cast<int>(cnt)};
       deconv[d].resize(nf1[d]);
       const double a2 = 0.5 * w * h2[d];
       for (int i = 0; i < nf1[d]; ++i) {
@@ -513,22 +530,28 @@ struct N3Plan {
       }
     }
   }
-  int n_bins() const { return nbins[0] * nbins[1] * nbins[2]; }
+  long long n_bins() const { return (long long)nbins[0] * nbins[1] * nbins[2]; }
 };
 
 // Runs the NUFFT image stage for one batch: points from positions, values from
 // VV_l = Z diag(w_l) Z^H, all levels accumulated into `img` (TI per pixel).
 template <typename TG>
 struct N3Runner {
-  DevBuf grid, valid, bin_of, l0x, l0y, l0z, frx, fry, frz, phr, phi, mult, src;
-  DevBuf bin_count, bin_start, bin_fill, scan_tmp, sorted, vv, deconv, gl;
+  DevBuf grid, valid, bin_of, pts, sorted;
+  DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid;
+  std::vector<cplx<TG>> htw;
+  std::vector<TG> hdec;
 
   template <typename RE>
   void run(ContextImpl& c, const N3Plan& pl, int nS, int L, int n_cols, int rows_max,
            const int* n_rows, const cplx<RE>* z, long long stride_z, int ldz, const RE* wd,
            const double* pos, long long pos_stride, int apad, const double* lmn64, int n_pix,
-           void* img, bool phase_fp32) {
+           void* img, bool phase_fp32, hipEvent_t* evs) {
     hipStream_t st = c.stream;
+    auto mark = [&](int i) {
+      if (evs) BIPP_HIP(hipEventRecord(evs[i], st));
+    };
+    mark(0);
     // virtual visibilities of every (sample, level), lower triangle
     const long long ldv = rows_max;
     const long long vv_stride = ldv * ldv;
@@ -546,10 +569,12 @@ struct N3Runner {
     vp.vv_stride = vv_stride;
     vp.ld_vv = ldv;
     launch_vv_levels<RE, TG>(vp, nS, rows_max, st);
+    mark(1);
 
     // points, bins, sort
     const long long npair = (long long)rows_max * (rows_max + 1) / 2;
     const long long npts = npair * nS;
+    if (npts > 0xffffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT points in one batch");
     N3PointsParams pp;
     pp.n_ant_max = rows_max;
     pp.n_rows = n_rows;
@@ -570,129 +595,170 @@ struct N3Runner {
     pp.ld_vv = ldv;
     pp.valid = valid.as<unsigned char>(npts);
     pp.bin_of = bin_of.as<int>(npts);
-    pp.l0x = l0x.as<int>(npts);
-    pp.l0y = l0y.as<int>(npts);
-    pp.l0z = l0z.as<int>(npts);
-    pp.frx = frx.as<float>(npts);
-    pp.fry = fry.as<float>(npts);
-    pp.frz = frz.as<float>(npts);
-    pp.phr = phr.as<float>(npts);
-    pp.phi = phi.as<float>(npts);
-    pp.mult = mult.as<float>(npts);
-    pp.src = src.as<int>(npts);
-    const int nb = pl.n_bins();
+    pp.pts = pts.as<N3Point>(npts);
+    const long long nb = pl.n_bins();
+    if (nb > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT bins");
     pp.bin_count = bin_count.as<unsigned>(nb);
     launch_zero<unsigned>(pp.bin_count, nb, st);
     launch_n3_points<double>(pp, nS, st);
     unsigned* bstart = bin_start.as<unsigned>(nb);
     unsigned* btmp = scan_tmp.as<unsigned>(3 * ((size_t)nb / 1024 + 2) + 4096);
-    launch_scan(pp.bin_count, bstart, btmp, nb, st);
+    launch_scan(pp.bin_count, bstart, btmp, (int)nb, st);
     unsigned* bfill = bin_fill.as<unsigned>(nb);
     launch_zero<unsigned>(bfill, nb, st);
-    unsigned* srt = sorted.as<unsigned>(npts);
-    launch_n3_scatter(npts, pp.valid, pp.bin_of, bstart, bfill, srt, st);
+    N3Point* srt = sorted.as<N3Point>(npts);
+    launch_n3_scatter(npts, pp.valid, pp.bin_of, pp.pts, bstart, bfill, srt, st);
+    mark(2);
 
-    // grid: zero, spread, deconvolve
+    // spread + deconvolution into [-nf1/2, nf1/2)^3 of the nf2 grid
     const long long G = pl.grid_cells;
-    TG* gp = grid.as<TG>((size_t)2 * L * G);
-    launch_zero<TG>(gp, 2LL * L * G, st);
-    N3SpreadParams<TG> sp;
-    sp.n_levels = L;
-    sp.w = pl.w;
-    sp.beta = pl.beta;
-    for (int d = 0; d < 3; ++d) {
-      sp.bin_size[d] = pl.bin[d];
-      sp.nbins[d] = pl.nbins[d];
-      sp.off[d] = pl.off[d];
-      sp.nf2[d] = pl.nf2[d];
-    }
-    sp.bin_start = bstart;
-    sp.bin_count = pp.bin_count;
-    sp.sorted = srt;
-    sp.l0x = pp.l0x;
-    sp.l0y = pp.l0y;
-    sp.l0z = pp.l0z;
-    sp.src = pp.src;
-    sp.frx = pp.frx;
-    sp.fry = pp.fry;
-    sp.frz = pp.frz;
-    sp.phr = pp.phr;
-    sp.phi = pp.phi;
-    sp.mult = pp.mult;
-    sp.npair = npair;
-    sp.vv = vvp;
-    sp.vv_stride = vv_stride;
-    sp.grid = gp;
-    sp.grid_stride = G;
-    launch_n3_spread<TG, cplx<TG>>(sp, nb, st);
-
+    cplx<TG>* cg = grid.as<cplx<TG>>((size_t)L * G);
     const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
-    std::vector<TG> hdec(nd);
+    hdec.resize(nd);
     for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
     for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
     for (int i = 0; i < pl.nf1[2]; ++i)
       hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
     TG* ddec = deconv.as<TG>(nd);
     BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
-    N3DeconvParams<TG> dp;
+    N3SpreadParams<TG> sp;
+    sp.n_levels = L;
+    sp.lv0 = 0;
+    sp.n_levels_total = L;
+    sp.w = pl.w;
+    sp.beta = pl.beta;
     for (int d = 0; d < 3; ++d) {
-      dp.nf1[d] = pl.nf1[d];
-      dp.nf2[d] = pl.nf2[d];
+      sp.bin_size[d] = pl.bin[d];
+      sp.nbins[d] = pl.nbins[d];
+      sp.nback[d] = pl.nback[d];
+      sp.nf1[d] = pl.nf1[d];
+      sp.nf2[d] = pl.nf2[d];
     }
-    dp.fx = ddec;
-    dp.fy = ddec + pl.nf1[0];
-    dp.fz = ddec + pl.nf1[0] + pl.nf1[1];
-    dp.grid = gp;
-    dp.grid_stride = G;
-    launch_n3_deconv<TG>(dp, L, st);
+    sp.bin_start = bstart;
+    sp.bin_count = pp.bin_count;
+    sp.pts = srt;
+    sp.vv = vvp;
+    sp.vv_stride = vv_stride;
+    sp.fx = ddec;
+    sp.fy = ddec + pl.nf1[0];
+    sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
+    sp.grid = cg;
+    sp.grid_stride = G;
+    launch_n3_spread<TG, cplx<TG>>(sp, st);
+    mark(3);
 
-    // FFT (sign +) along x, y, z
-    cplx<TG>* cg = reinterpret_cast<cplx<TG>*>(gp);
-    const long long nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
-    launch_fft_x<TG>(cg, (int)nx, (long long)L * ny * nz, radices_of((int)nx), st);
-    for (int d = 1; d < 3; ++d) {
-      N3StridedParams<TG> fp;
-      fp.grid = cg;
-      if (d == 1) {
-        fp.nx = nx;
-        fp.axis_stride = nx;
-        fp.n_outer = (long long)L * nz;
-        fp.outer_stride = nx * ny;
-      } else {
-        fp.nx = nx * ny;
-        fp.axis_stride = nx * ny;
-        fp.n_outer = L;
-        fp.outer_stride = G;
+    // pruned FFT (sign +) along x, y, z; twiddle tables exp(+2 pi i t / n) computed in FP64
+    const int nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
+    // table segments: x, then per strided axis n1, n2 and n1 * n2
+    size_t tw_off[3][3] = {};
+    htw.clear();
+    auto add_table = [&](long long n) {
+      const size_t o = htw.size();
+      for (long long t = 0; t < n; ++t) {
+        const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t) / n;
+        htw.push_back(mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang))));
       }
+      return o;
+    };
+    tw_off[0][0] = add_table(nx);
+    for (int d = 1; d < 3; ++d) {
+      tw_off[d][0] = add_table(pl.n1[d]);
+      tw_off[d][1] = add_table(pl.n2[d]);
+      tw_off[d][2] = pl.n2[d] > 1 ? add_table((long long)pl.n1[d] * pl.n2[d]) : 0;
+    }
+    cplx<TG>* dtw = twid.as<cplx<TG>>(htw.size());
+    BIPP_HIP(hipMemcpyAsync(dtw, htw.data(), sizeof(cplx<TG>) * htw.size(),
+                            hipMemcpyHostToDevice, st));
+    // x: lines (y, z) of the input window; read x of the input window, write the output window
+    N3LineParams<TG> lp;
+    lp.grid = cg;
+    lp.grid_stride = G;
+    lp.n_levels = L;
+    lp.n = nx;
+    lp.ny = ny;
+    lp.nz = nz;
+    lp.yw = pl.in_w[1];
+    lp.zw = pl.in_w[2];
+    lp.in = pl.in_w[0];
+    lp.out = pl.out_w[0];
+    lp.rad = radices_of(nx);
+    lp.tw = dtw + tw_off[0][0];
+    launch_fft_lines<TG>(lp, st);
+    // y over (x of the output window, z of the input window); z over (x, y output windows)
+    for (int d = 1; d < 3; ++d) {
+      N3PassParams<TG> fp{};
+      fp.grid = cg;
+      fp.grid_stride = G;
+      fp.n_levels = L;
+      fp.nx = nx;
+      fp.xw = pl.out_w[0];
+      if (d == 1) {
+        fp.no = nz;
+        fp.other = pl.in_w[2];
+        fp.operm_n1 = 1;
+        fp.operm_n2 = 1;
+        fp.outer_stride = (long long)nx * ny;
+        fp.axis_stride = nx;
+      } else {
+        fp.no = ny;
+        fp.other = pl.out_w[1];
+        fp.operm_n1 = pl.n1[1];
+        fp.operm_n2 = pl.n2[1];
+        fp.outer_stride = nx;
+        fp.axis_stride = (long long)nx * ny;
+      }
+      fp.nax = pl.nf2[d];
+      fp.in = pl.in_w[d];
+      fp.out = pl.out_w[d];
       const int n1 = pl.n1[d], n2 = pl.n2[d];
       if (n2 == 1) {
         fp.m = n1;
-        fp.estride = 1;
-        fp.sub_stride = 0;
         fp.n_sub = 1;
+        fp.sub_stride = 0;
+        fp.estride = 1;
+        fp.in_check = 1;
+        fp.out_check = 1;
+        fp.fa = 0;
+        fp.fb = 1;
         fp.twiddle = 0;
         fp.ntot = n1;
         fp.rad = radices_of(n1);
-        launch_fft_strided<TG>(fp, st);
+        fp.tw = dtw + tw_off[d][0];
+        fp.tw2 = nullptr;
+        launch_fft_pass<TG>(fp, st);
       } else {
+        // n = n1 n2: transforms of length n1 over stride n2 (inputs pruned), twiddle, then
+        // transforms of length n2 over contiguous runs (outputs pruned); frequency k is
+        // stored at (k mod n1) n2 + k / n1
         fp.m = n1;
-        fp.estride = n2;
-        fp.sub_stride = 1;
         fp.n_sub = n2;
+        fp.sub_stride = 1;
+        fp.estride = n2;
+        fp.in_check = 1;
+        fp.out_check = 0;
         fp.twiddle = 1;
         fp.ntot = (long long)n1 * n2;
         fp.rad = radices_of(n1);
-        launch_fft_strided<TG>(fp, st);
+        fp.tw = dtw + tw_off[d][0];
+        fp.tw2 = dtw + tw_off[d][2];
+        launch_fft_pass<TG>(fp, st);
         fp.m = n2;
-        fp.estride = 1;
-        fp.sub_stride = n2;
         fp.n_sub = n1;
+        fp.sub_stride = n2;
+        fp.estride = 1;
+        fp.in_check = 0;
+        fp.out_check = 1;
+        fp.fa = 1;
+        fp.fb = n1;
         fp.twiddle = 0;
         fp.rad = radices_of(n2);
-        launch_fft_strided<TG>(fp, st);
+        fp.tw = dtw + tw_off[d][1];
+        fp.tw2 = nullptr;
+        launch_fft_pass<TG>(fp, st);
       }
     }
 
+    mark(4);
     // interpolation, correction, real part into the images
     double* dgl = gl.as<double>(2 * pl.gl_x.size());
     BIPP_HIP(hipMemcpyAsync(dgl, pl.gl_x.data(), sizeof(double) * pl.gl_x.size(),
@@ -726,6 +792,7 @@ struct N3Runner {
     ip.scale = 1.0;
     ip.img_ld = n_pix;
     launch_n3_interp<TG, TG>(ip, st);
+    mark(5);
   }
 };
 
@@ -761,14 +828,21 @@ struct BippSynthesisImpl {
   std::vector<int32_t> last_nred;
   int last_count = 0;
   // stage timings on the context stream: 0 eigensolve, 1 prep/weights, 2 image, 3 launches
+  // NUFFT sub-stages: 4 virtual vis, 5 point sort, 6 spread, 7 deconvolve + FFT, 8 interpolate
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double stage_ms[4] = {0, 0, 0, 0};
+  hipEvent_t ev_n3[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  static constexpr int N_TIMES = 9;
+  double stage_ms[N_TIMES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool n3_timed = false;
 
   void init_events() {
     for (auto& e : ev) BIPP_HIP(hipEventCreate(&e));
+    for (auto& e : ev_n3) BIPP_HIP(hipEventCreate(&e));
   }
   ~BippSynthesisImpl() {
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_n3)
       if (e) (void)hipEventDestroy(e);
   }
   void accumulate_times() {
@@ -780,6 +854,13 @@ struct BippSynthesisImpl {
     BIPP_HIP(hipEventElapsedTime(&t, ev[2], ev[3]));
     stage_ms[2] += t;
     stage_ms[3] += 1;
+    if (n3_timed) {
+      for (int i = 0; i < 5; ++i) {
+        BIPP_HIP(hipEventElapsedTime(&t, ev_n3[i], ev_n3[i + 1]));
+        stage_ms[4 + i] += t;
+      }
+      n3_timed = false;
+    }
   }
 
   bool img_double() const { return compute == BIPP_COMPUTE_FP64; }
@@ -933,11 +1014,12 @@ struct BippSynthesisImpl {
       if (compute == BIPP_COMPUTE_FP64)
         n3d.run<R>(c, n3plan, nS, n_levels, n_beam, rows_max, pp.n_ant, pp.z, pp.stride_z, pp.ldz,
                    wd, pp.pos, pp.pos_stride, apad, static_cast<const double*>(lmn64.p), n_pix,
-                   img.p, false);
+                   img.p, false, ev_n3);
       else
         n3f.run<R>(c, n3plan, nS, n_levels, n_beam, rows_max, pp.n_ant, pp.z, pp.stride_z, pp.ldz,
                    wd, pp.pos, pp.pos_stride, apad, static_cast<const double*>(lmn64.p), n_pix,
-                   img.p, compute == BIPP_COMPUTE_FP32);
+                   img.p, compute == BIPP_COMPUTE_FP32, ev_n3);
+      n3_timed = true;
       BIPP_HIP(hipEventRecord(ev[3], st));
     } else {
       // virtual visibilities + direct NUFT sum, per sample and level
@@ -1412,8 +1494,8 @@ BippError bipp_synthesis_nufft_grid(BippSynthesis h, int32_t* nf) {
 BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset) {
   return guard([&] {
     require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
-    if (ms) std::copy(h->stage_ms, h->stage_ms + 4, ms);
-    if (reset) std::fill(h->stage_ms, h->stage_ms + 4, 0.0);
+    if (ms) std::copy(h->stage_ms, h->stage_ms + BippSynthesisImpl::N_TIMES, ms);
+    if (reset) std::fill(h->stage_ms, h->stage_ms + BippSynthesisImpl::N_TIMES, 0.0);
   });
 }
 

@@ -407,7 +407,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
           R* e = sh.ee;
           R g = d[l];
           R pp = (d[l + 1] - g) / (R(2) * e[l]);
-          R r = hypot(pp, R(1));
+          R r = sqrt(pp * pp + R(1));
           if (pp < 0) r = -r;
           d[l] = e[l] / (pp + r);
           d[l + 1] = e[l] * (pp + r);
@@ -424,10 +424,18 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
             s2 = s;
             g = c * e[i];
             h = c * pp;
-            r = hypot(pp, e[i]);
+            // Givens rotation zeroing e[i]: one square root and one reciprocal
+            const R ei = e[i];
+            r = sqrt(pp * pp + ei * ei);
             e[i + 1] = s * r;
-            s = e[i] / r;
-            c = pp / r;
+            if (r != R(0)) {
+              const R rinv = R(1) / r;
+              s = ei * rinv;
+              c = pp * rinv;
+            } else {
+              s = R(0);
+              c = R(1);
+            }
             pp = c * d[i] - s * g;
             d[i + 1] = h + s * (c * g + s * d[i]);
             sh.cs[i] = c;
@@ -439,14 +447,26 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
           sh.iscal[3] = !(fabs(e[l]) > eps * sh.scal[2]);
         }
         __syncthreads();
-        // apply rotations i = m-1 .. l to the rows of Z owned by this thread
+        // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
+        // fetched 16 at a time so that their loads are in flight together
+        constexpr int CH = 16;
         for (int row = tid; row < n; row += NT) {
           R carry = Z[row + (size_t)m * ld];
-          for (int i = m - 1; i >= l; --i) {
-            const R zi = Z[row + (size_t)i * ld];
-            const R c = sh.cs[i], s = sh.sn[i];
-            Z[row + (size_t)(i + 1) * ld] = s * zi + c * carry;
-            carry = c * zi - s * carry;
+          for (int i0 = m - 1; i0 >= l; i0 -= CH) {
+            const int cnt = min(CH, i0 - l + 1);
+            R zc[CH];
+#pragma unroll
+            for (int t = 0; t < CH; ++t)
+              if (t < cnt) zc[t] = Z[row + (size_t)(i0 - t) * ld];
+#pragma unroll
+            for (int t = 0; t < CH; ++t) {
+              if (t < cnt) {
+                const int i = i0 - t;
+                const R c = sh.cs[i], s = sh.sn[i];
+                Z[row + (size_t)(i + 1) * ld] = s * zc[t] + c * carry;
+                carry = c * zc[t] - s * carry;
+              }
+            }
           }
           Z[row + (size_t)l * ld] = carry;
         }

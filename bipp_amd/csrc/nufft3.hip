@@ -71,8 +71,8 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   }
   const double* pos = p.pos + (size_t)s * p.pos_stride;  // [3][apad] positions / wavelength
   double u[3], cell[3];
-  int l0[3], bin[3];
-  float fr[3];
+  N3Point pt;
+  int bin[3];
   double ph = 0;
   for (int d = 0; d < 3; ++d) {
     u[d] = 6.283185307179586 * (pos[d * p.apad + a2] - pos[d * p.apad + b]);
@@ -80,9 +80,12 @@ __global__ void n3_points_kernel(N3PointsParams p) {
     ph += up * p.D[d];
     cell[d] = up * p.inv_gh[d];  // t / h1 in grid units
     if (p.phase_fp32) cell[d] = static_cast<double>(static_cast<float>(up) * static_cast<float>(p.inv_gh[d]));
-    l0[d] = static_cast<int>(ceil(cell[d] - 0.5 * p.w));
-    fr[d] = static_cast<float>(cell[d] - l0[d]);
-    bin[d] = (l0[d] + p.off[d]) / p.bin_size[d];
+    const int l0 = static_cast<int>(ceil(cell[d] - 0.5 * p.w));
+    pt.fr[d] = static_cast<float>(cell[d] - l0);
+    pt.l0[d] = l0 + p.off[d];
+    // the plan keeps every support inside [-nf1/2, nf1/2); the clamp only keeps the bin
+    // index in range (cells outside a block are never written by it)
+    bin[d] = min(max(pt.l0[d], 0) / p.bin_size[d], p.nbins[d] - 1);
   }
   double ps, pc;
   if (p.phase_fp32) {
@@ -99,18 +102,15 @@ __global__ void n3_points_kernel(N3PointsParams p) {
     sincospi(2.0 * f, &ps, &pc);
   }
   const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
+  pt.ph[0] = static_cast<float>(pc);
+  pt.ph[1] = static_cast<float>(ps);
+  pt.src = static_cast<int>(a2 + p.ld_vv * b);  // VV(a, b) inside the sample's matrix
+  pt.mult = a2 == b ? 1.f : 2.f;                 // (a,b) and (b,a) are conjugate: 2 Re(.)
+  pt.sample = s;
+  pt.pad = 0;
   p.valid[pid] = 1;
   p.bin_of[pid] = binid;
-  p.l0x[pid] = l0[0];
-  p.l0y[pid] = l0[1];
-  p.l0z[pid] = l0[2];
-  p.frx[pid] = fr[0];
-  p.fry[pid] = fr[1];
-  p.frz[pid] = fr[2];
-  p.phr[pid] = static_cast<float>(pc);
-  p.phi[pid] = static_cast<float>(ps);
-  p.src[pid] = a2 + p.ld_vv * b;  // VV(a, b) inside the sample's matrix
-  p.mult[pid] = a2 == b ? 1.f : 2.f;  // (a,b) and (b,a) are conjugate: 2 Re(.)
+  p.pts[pid] = pt;
   atomicAdd(&p.bin_count[binid], 1u);
 }
 
@@ -136,122 +136,146 @@ __global__ void scan_add_kernel(unsigned* out, const unsigned* block_offsets, in
   if (i < n) out[i] += block_offsets[blockIdx.x];
 }
 
+// counting sort: point records copied to their bin's segment
 __global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, const int* bin_of,
-                                  const unsigned* bin_start, unsigned* bin_fill, unsigned* sorted) {
+                                  const N3Point* pts, const unsigned* bin_start,
+                                  unsigned* bin_fill, N3Point* sorted) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n_pts || !valid[i]) return;
   const int b = bin_of[i];
   const unsigned slot = bin_start[b] + atomicAdd(&bin_fill[b], 1u);
-  sorted[slot] = static_cast<unsigned>(i);
+  sorted[slot] = pts[i];
 }
 
 // --------------------------------------------------------------------------------------
-// Spreading: one workgroup per bin, LDS tile (bin + kernel width) for every level.
-// Threads own kernel offsets (ox, oy, oz); points are taken in batches whose kernel
-// weights are computed once into LDS.
+// Spreading with output ownership.  The workgroup of block (bx, by, bz) owns the cells
+// l + nf1/2 in [b B, b B + B) per axis.  It walks the bins that can reach the block (a point
+// of bin c covers cells [l0, l0 + w), l0 in bin c), clips each point's support to the block,
+// accumulates kernel weight x value for every level in an LDS tile, and finally writes the
+// tile times the deconvolution factors with plain stores.  Every cell of [-nf1/2, nf1/2)^3
+// is written by exactly one workgroup; no global atomics, no zero pass over the grid.
 // --------------------------------------------------------------------------------------
+constexpr int SP_PB = 64;  // points per batch (one wavefront scans the item counts)
+
 template <typename R, typename TV>
 __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p) {
-  extern __shared__ unsigned char sp_sh[];
-  const int bin = blockIdx.x;
-  const unsigned beg = p.bin_start[bin];
-  const unsigned cnt = p.bin_count[bin];
-  if (cnt == 0) return;
+  extern __shared__ __align__(16) unsigned char sp_sh[];
   const int w = p.w;
-  const int TX = p.bin_size[0] + w, TY = p.bin_size[1] + w, TZ = p.bin_size[2] + w;
-  const int tcells = TX * TY * TZ;
+  const int BX = p.bin_size[0], BY = p.bin_size[1], BZ = p.bin_size[2];
+  const int tcells = BX * BY * BZ;
   const int L = p.n_levels;
-  R* tile = reinterpret_cast<R*>(sp_sh);                     // [L][tcells][2]
-  constexpr int PB = 32;                                     // points per batch
-  R* wts = tile + (size_t)2 * L * tcells;                    // [PB][3][w]
-  R* val = wts + (size_t)PB * 3 * w;                         // [PB][L][2]
-  int* org = reinterpret_cast<int*>(val + (size_t)PB * L * 2);  // [PB][3] tile-local origin
+  R* tile = reinterpret_cast<R*>(sp_sh);                          // [L][tcells][2]
+  R* wts = tile + (size_t)2 * L * tcells;                         // [PB][3][w]
+  R* val = wts + (size_t)SP_PB * 3 * w;                           // [PB][L][2]
+  int* info = reinterpret_cast<int*>(val + (size_t)SP_PB * L * 2);  // [PB][8]
+  int* pref = info + SP_PB * 8;                                   // [PB + 1]
+  const int bin = blockIdx.x;
   const int bx = bin % p.nbins[0];
   const int by = (bin / p.nbins[0]) % p.nbins[1];
   const int bz = bin / (p.nbins[0] * p.nbins[1]);
-  const int ox0 = bx * p.bin_size[0], oy0 = by * p.bin_size[1], oz0 = bz * p.bin_size[2];
+  const int org[3] = {bx * BX, by * BY, bz * BZ};
+  const int bsz[3] = {BX, BY, BZ};
   for (int i = threadIdx.x; i < 2 * L * tcells; i += SP_NT) tile[i] = R(0);
-  const int w3 = w * w * w;
   const R two_over_w = R(2) / R(w);
-  __syncthreads();
-  for (unsigned b0 = 0; b0 < cnt; b0 += PB) {
-    const int nb = min<unsigned>(PB, cnt - b0);
-    // kernel weights and values of this batch
-    for (int t = threadIdx.x; t < nb * 3 * w; t += SP_NT) {
-      const int pi = t / (3 * w), rem = t % (3 * w), d = rem / w, o = rem % w;
-      const unsigned id = p.sorted[beg + b0 + pi];
-      const float fr = d == 0 ? p.frx[id] : (d == 1 ? p.fry[id] : p.frz[id]);
-      wts[(pi * 3 + d) * w + o] = es_kernel<R>((R(o) - R(fr)) * two_over_w, R(p.beta));
-    }
-    for (int t = threadIdx.x; t < nb * L; t += SP_NT) {
-      const int pi = t / L, lv = t % L;
-      const unsigned id = p.sorted[beg + b0 + pi];
-      const long long s = id / p.npair;
-      const TV* vv = p.vv + ((size_t)s * L + lv) * p.vv_stride;
-      const TV c = vv[p.src[id]];
-      const R m = R(p.mult[id]);
-      const R pr = R(p.phr[id]), pim = R(p.phi[id]);
-      val[(pi * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
-      val[(pi * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
-    }
-    for (int t = threadIdx.x; t < nb; t += SP_NT) {
-      const unsigned id = p.sorted[beg + b0 + t];
-      org[t * 3 + 0] = p.l0x[id] + p.off[0] - ox0;
-      org[t * 3 + 1] = p.l0y[id] + p.off[1] - oy0;
-      org[t * 3 + 2] = p.l0z[id] + p.off[2] - oz0;
-    }
-    __syncthreads();
-    // work item = (point, kernel offset); consecutive lanes take consecutive offsets of one
-    // point, i.e. distinct tile cells
-    for (int it = threadIdx.x; it < nb * w3; it += SP_NT) {
-      const int pi = it / w3, off = it % w3;
-      const int oxo = off % w, oyo = (off / w) % w, ozo = off / (w * w);
-      const R wt = wts[(pi * 3 + 0) * w + oxo] * wts[(pi * 3 + 1) * w + oyo] *
-                   wts[(pi * 3 + 2) * w + ozo];
-      const int cx = org[pi * 3 + 0] + oxo, cy = org[pi * 3 + 1] + oyo, cz = org[pi * 3 + 2] + ozo;
-      const int cell = (cz * TY + cy) * TX + cx;
-      for (int lv = 0; lv < L; ++lv) {
-        atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 0], wt * val[(pi * L + lv) * 2 + 0]);
-        atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 1], wt * val[(pi * L + lv) * 2 + 1]);
+  const R beta = R(p.beta);
+  for (int cz = max(0, bz - p.nback[2]); cz <= bz; ++cz)
+    for (int cy = max(0, by - p.nback[1]); cy <= by; ++cy)
+      for (int cx = max(0, bx - p.nback[0]); cx <= bx; ++cx) {
+        const int nbid = (cz * p.nbins[1] + cy) * p.nbins[0] + cx;
+        const unsigned beg = p.bin_start[nbid];
+        const unsigned cnt = p.bin_count[nbid];
+        for (unsigned b0 = 0; b0 < cnt; b0 += SP_PB) {
+          const int nb = min<unsigned>(SP_PB, cnt - b0);
+          __syncthreads();  // previous batch fully consumed
+          // support of each point clipped to the block
+          if (threadIdx.x < SP_PB) {
+            int items = 0;
+            if ((int)threadIdx.x < nb) {
+              const N3Point& pt = p.pts[beg + b0 + threadIdx.x];
+              items = 1;
+              for (int d = 0; d < 3; ++d) {
+                const int lo = max(pt.l0[d], org[d]);
+                const int hi = min(pt.l0[d] + w, org[d] + bsz[d]);
+                const int c = max(0, hi - lo);
+                info[threadIdx.x * 8 + d] = lo - pt.l0[d];      // first kernel offset
+                info[threadIdx.x * 8 + 3 + d] = c;              // offsets inside the block
+                items *= c;
+              }
+              info[threadIdx.x * 8 + 6] = pt.l0[0] - org[0];
+              info[threadIdx.x * 8 + 7] = (pt.l0[1] - org[1]) * BX + (pt.l0[2] - org[2]) * BX * BY;
+            }
+            // inclusive scan over the wavefront
+            int v = items;
+            for (int o = 1; o < SP_PB; o <<= 1) {
+              const int u = __shfl_up(v, o, SP_PB);
+              if ((int)threadIdx.x >= o) v += u;
+            }
+            pref[threadIdx.x + 1] = v;
+            if (threadIdx.x == 0) pref[0] = 0;
+          }
+          __syncthreads();
+          const int total = pref[nb];
+          if (total == 0) continue;
+          // kernel weights of the clipped offsets, values of every level
+          for (int t = threadIdx.x; t < nb * 3 * w; t += SP_NT) {
+            const int pi = t / (3 * w), rem = t % (3 * w), d = rem / w, j = rem % w;
+            if (j >= info[pi * 8 + 3 + d]) continue;
+            const N3Point& pt = p.pts[beg + b0 + pi];
+            const int o = info[pi * 8 + d] + j;
+            wts[(pi * 3 + d) * w + j] = es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
+          }
+          for (int t = threadIdx.x; t < nb * L; t += SP_NT) {
+            const int pi = t / L, lv = t % L;
+            if (pref[pi + 1] == pref[pi]) continue;
+            const N3Point& pt = p.pts[beg + b0 + pi];
+            const TV* vv =
+                p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0 + lv) * p.vv_stride;
+            const TV c = vv[pt.src];
+            const R m = R(pt.mult);
+            const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
+            val[(pi * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
+            val[(pi * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+          }
+          __syncthreads();
+          // items (point, clipped offset), x offsets fastest: consecutive lanes take
+          // consecutive cells
+          for (int it = threadIdx.x; it < total; it += SP_NT) {
+            int lo = 0, hi = nb;  // pref[lo] <= it < pref[hi]
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (pref[mid] <= it) lo = mid;
+              else hi = mid;
+            }
+            const int pi = lo;
+            const int r = it - pref[pi];
+            const int c0 = info[pi * 8 + 3], c1 = info[pi * 8 + 4];
+            const int jx = r % c0, rq = r / c0, jy = rq % c1, jz = rq / c1;
+            const R wt = wts[(pi * 3 + 0) * w + jx] * wts[(pi * 3 + 1) * w + jy] *
+                         wts[(pi * 3 + 2) * w + jz];
+            // tile cell of kernel offset (info0 + jx, info1 + jy, info2 + jz)
+            const int cell = info[pi * 8 + 6] + info[pi * 8 + 0] + jx +
+                             (info[pi * 8 + 1] + jy) * BX + (info[pi * 8 + 2] + jz) * BX * BY +
+                             info[pi * 8 + 7];
+            for (int lv = 0; lv < L; ++lv) {
+              atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 0], wt * val[(pi * L + lv) * 2 + 0]);
+              atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 1], wt * val[(pi * L + lv) * 2 + 1]);
+            }
+          }
+        }
       }
-    }
-    __syncthreads();
-  }
-  // flush the tile into the nf2 grid (spread-grid index l -> l mod nf2)
+  __syncthreads();
+  // deconvolve and store the owned cells inside [-nf1/2, nf1/2)^3 (grid index l mod nf2)
   for (int i = threadIdx.x; i < L * tcells; i += SP_NT) {
     const int lv = i / tcells, cell = i % tcells;
-    const R re = tile[(size_t)i * 2], im = tile[(size_t)i * 2 + 1];
-    if (re == R(0) && im == R(0)) continue;
-    const int cx = cell % TX, cy = (cell / TX) % TY, cz = cell / (TX * TY);
-    int gx = ox0 + cx - p.off[0], gy = oy0 + cy - p.off[1], gz = oz0 + cz - p.off[2];
-    gx = ((gx % p.nf2[0]) + p.nf2[0]) % p.nf2[0];
-    gy = ((gy % p.nf2[1]) + p.nf2[1]) % p.nf2[1];
-    gz = ((gz % p.nf2[2]) + p.nf2[2]) % p.nf2[2];
-    R* g = p.grid + ((size_t)lv * p.grid_stride + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx) * 2;
-    atomicAdd(g, re);
-    atomicAdd(g + 1, im);
-  }
-}
-
-// --------------------------------------------------------------------------------------
-// Deconvolution of the spread modes l in [-nf1/2, nf1/2) by 1/phi2^(l).
-// --------------------------------------------------------------------------------------
-template <typename R>
-__global__ void n3_deconv_kernel(N3DeconvParams<R> p) {
-  const long long n = (long long)p.nf1[0] * p.nf1[1] * p.nf1[2];
-  const int lv = blockIdx.y;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int ix = static_cast<int>(t % p.nf1[0]);
-    const int iy = static_cast<int>((t / p.nf1[0]) % p.nf1[1]);
-    const int iz = static_cast<int>(t / ((long long)p.nf1[0] * p.nf1[1]));
-    const int lx = ix - p.nf1[0] / 2, ly = iy - p.nf1[1] / 2, lz = iz - p.nf1[2] / 2;
+    const int ix = org[0] + cell % BX, iy = org[1] + (cell / BX) % BY, iz = org[2] + cell / (BX * BY);
+    if (ix >= p.nf1[0] || iy >= p.nf1[1] || iz >= p.nf1[2]) continue;
     const R f = p.fx[ix] * p.fy[iy] * p.fz[iz];
-    const int gx = (lx + p.nf2[0]) % p.nf2[0], gy = (ly + p.nf2[1]) % p.nf2[1],
-              gz = (lz + p.nf2[2]) % p.nf2[2];
-    R* g = p.grid + ((size_t)lv * p.grid_stride + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx) * 2;
-    g[0] *= f;
-    g[1] *= f;
+    int gx = ix - p.nf1[0] / 2, gy = iy - p.nf1[1] / 2, gz = iz - p.nf1[2] / 2;
+    gx += gx < 0 ? p.nf2[0] : 0;
+    gy += gy < 0 ? p.nf2[1] : 0;
+    gz += gz < 0 ? p.nf2[2] : 0;
+    p.grid[(size_t)(p.lv0 + lv) * p.grid_stride + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx] =
+        mk<R>(tile[(size_t)i * 2] * f, tile[(size_t)i * 2 + 1] * f);
   }
 }
 
@@ -259,6 +283,7 @@ __global__ void n3_deconv_kernel(N3DeconvParams<R> p) {
 // Stockham FFT in LDS (sign +): `nl` lines of length n, line stride ld, radices r[].
 // Returns the buffer holding the result.
 // --------------------------------------------------------------------------------------
+// Small DFTs with sign +, constant roots.
 template <typename R>
 __device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
   if (rad == 2) {
@@ -273,50 +298,56 @@ __device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
     v[1] = s1 + is3;
     v[2] = s0 - s2;
     v[3] = s1 - is3;
+  } else if (rad == 3) {
+    // w = exp(+2 pi i / 3): y1,2 = a - (b + c)/2 +- i sqrt(3)/2 (b - c)
+    const R h = R(0.86602540378443864676);
+    const cplx<R> a = v[0], t1 = v[1] + v[2], t2 = v[1] - v[2];
+    const cplx<R> m = mk<R>(a.re - R(0.5) * t1.re, a.im - R(0.5) * t1.im);
+    const cplx<R> q = mk<R>(-h * t2.im, h * t2.re);
+    v[0] = a + t1;
+    v[1] = m + q;
+    v[2] = m - q;
   } else {
+    // radix 5: roots exp(+2 pi i k / 5)
+    const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
+    const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
+    const R cr[5] = {R(1), c1, c2, c2, c1};
+    const R sr[5] = {R(0), s1, s2, -s2, -s1};
     cplx<R> o[5];
-    for (int k = 0; k < rad; ++k) {
-      cplx<R> acc = mk<R>(0, 0);
-      for (int r = 0; r < rad; ++r) {
-        R sn, cs;
-        if constexpr (sizeof(R) == 8)
-          sincospi(2.0 * ((r * k) % rad) / rad, &sn, &cs);
-        else
-          sincospif(2.0f * ((r * k) % rad) / rad, &sn, &cs);
-        acc = acc + v[r] * mk<R>(cs, sn);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      cplx<R> acc = v[0];
+#pragma unroll
+      for (int r = 1; r < 5; ++r) {
+        const int e = (r * k) % 5;
+        acc = acc + v[r] * mk<R>(cr[e], sr[e]);
       }
       o[k] = acc;
     }
-    for (int k = 0; k < rad; ++k) v[k] = o[k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = o[k];
   }
 }
 
+// Stockham stages in LDS; `tw` holds exp(+2 pi i t / n) for t < n.
 template <typename R>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
-                            int tid, int nt) {
+                            const cplx<R>* __restrict__ tw, int tid, int nt) {
   int ns = 1;
   cplx<R>* in = a;
   cplx<R>* out = b;
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
     const int nb = n / R_;
+    const int tstride = n / (ns * R_);  // twiddle k r / (ns R) of the full length
     for (int t = tid; t < nl * nb; t += nt) {
       const int line = t / nb, j = t % nb;
       const cplx<R>* src = in + (size_t)line * ld;
       cplx<R>* dst = out + (size_t)line * ld;
       const int k = j % ns;
       cplx<R> v[5];
-      R sn, cs;
-      if constexpr (sizeof(R) == 8)
-        sincospi(2.0 * k / (double)(ns * R_), &sn, &cs);
-      else
-        sincospif(2.0f * k / (float)(ns * R_), &sn, &cs);
-      const cplx<R> w1 = mk<R>(cs, sn);
-      cplx<R> tw = mk<R>(1, 0);
-      for (int r = 0; r < R_; ++r) {
-        v[r] = src[j + r * nb] * tw;
-        tw = tw * w1;
-      }
+      v[0] = src[j];
+      for (int r = 1; r < R_; ++r) v[r] = src[j + r * nb] * tw[k * r * tstride];
       dft_small<R>(v, R_);
       const int idx = (j / ns) * ns * R_ + k;
       for (int r = 0; r < R_; ++r) dst[idx + r * ns] = v[r];
@@ -330,67 +361,101 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
   return in;
 }
 
-// Contiguous axis (x): one line of length n per workgroup.
+// position of index a inside window [start, start + count) of an axis of length n
+__device__ __forceinline__ bool in_window(int a, const N3Window& wd, int n) {
+  int d = a - wd.start;
+  d += d < 0 ? n : 0;
+  return d < wd.count;
+}
+
+// Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).
 template <typename R>
-__global__ __launch_bounds__(256) void fft_x_kernel(cplx<R>* grid, int n, long long n_lines,
-                                                    N3Radix rad) {
-  extern __shared__ unsigned char fsh[];
+__global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int nl) {
+  extern __shared__ __align__(16) unsigned char fsh[];
+  __shared__ long long lbase[16];
+  const int n = p.n;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
-  cplx<R>* b = a + n;
-  for (long long line = blockIdx.x; line < n_lines; line += gridDim.x) {
-    cplx<R>* g = grid + line * n;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = g[i];
+  cplx<R>* b = a + (size_t)nl * n;
+  const long long n_lines = (long long)p.n_levels * p.zw.count * p.yw.count;
+  const long long n_groups = (n_lines + nl - 1) / nl;
+  for (long long grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const long long line0 = grp * nl;
+    const int nlg = static_cast<int>(min<long long>(nl, n_lines - line0));
+    if ((int)threadIdx.x < nlg) {
+      const long long line = line0 + threadIdx.x;
+      const int yo = static_cast<int>(line % p.yw.count);
+      const long long r = line / p.yw.count;
+      const int zo = static_cast<int>(r % p.zw.count);
+      const int lv = static_cast<int>(r / p.zw.count);
+      int y = p.yw.start + yo, z = p.zw.start + zo;
+      y -= y >= p.ny ? p.ny : 0;
+      z -= z >= p.nz ? p.nz : 0;
+      lbase[threadIdx.x] = (long long)lv * p.grid_stride + ((long long)z * p.ny + y) * n;
+    }
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, n, 1, n, rad, threadIdx.x, blockDim.x);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = res[i];
+    for (int i = threadIdx.x; i < nlg * n; i += blockDim.x) {
+      const int li = i / n, x = i - li * n;
+      a[i] = in_window(x, p.in, n) ? p.grid[lbase[li] + x] : mk<R>(0, 0);
+    }
+    __syncthreads();
+    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, p.tw, threadIdx.x, blockDim.x);
+    for (int i = threadIdx.x; i < nlg * n; i += blockDim.x) {
+      const int li = i / n, k = i - li * n;
+      if (in_window(k, p.out, n)) p.grid[lbase[li] + k] = res[i];
+    }
     __syncthreads();
   }
 }
 
-// Strided axis: transform of length m over elements axis_index = base + i * estride
-// (i < m), for TW consecutive x positions at once.  Optionally multiplies element k1 of
-// sub-transform j2 by exp(+2 pi i j2 k1 / ntot) (first pass of a two-pass transform).
+// Strided axis: transforms of length m for TW consecutive x positions at once; element i of
+// sub-transform `sub` at axis index sub * sub_stride + i * estride.  The LDS line stride is
+// odd so that the TW lines of a load fall into different banks.
 template <typename R>
-__global__ __launch_bounds__(256) void fft_strided_kernel(N3StridedParams<R> p) {
-  extern __shared__ unsigned char fsh[];
+__global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
+  extern __shared__ __align__(16) unsigned char fsh[];
   constexpr int TW = 16;
+  const int ld = p.m | 1;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
-  cplx<R>* b = a + TW * p.m;
-  // work item: (x tile, outer index over the other axis, sub-transform index)
-  const long long n_xt = (p.nx + TW - 1) / TW;
-  for (long long item = blockIdx.x; item < n_xt * p.n_outer * p.n_sub; item += gridDim.x) {
-    const long long xt = item % n_xt;
-    const long long rest = item / n_xt;
+  cplx<R>* b = a + TW * ld;
+  const int n_xt = (p.xw.count + TW - 1) / TW;
+  const long long n_items = (long long)n_xt * p.n_sub * p.other.count * p.n_levels;
+  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int xt = static_cast<int>(item % n_xt);
+    long long rest = item / n_xt;
     const int sub = static_cast<int>(rest % p.n_sub);
-    const long long outer = rest / p.n_sub;
-    const long long x0 = xt * TW;
-    // element i of the transform lives at axis index sub * sub_stride + i * estride
+    rest /= p.n_sub;
+    const int o = static_cast<int>(rest % p.other.count);
+    const int lv = static_cast<int>(rest / p.other.count);
+    int oo = p.other.start + o;
+    oo -= oo >= p.no ? p.no : 0;
+    if (p.operm_n2 > 1) oo = (oo % p.operm_n1) * p.operm_n2 + oo / p.operm_n1;
+    cplx<R>* base = p.grid + (size_t)lv * p.grid_stride + (size_t)oo * p.outer_stride;
     for (int t = threadIdx.x; t < TW * p.m; t += blockDim.x) {
       const int xi = t % TW, i = t / TW;
-      const long long x = x0 + xi;
-      const long long ai = (long long)sub * p.sub_stride + (long long)i * p.estride;
+      const int j = xt * TW + xi;
       cplx<R> v = mk<R>(0, 0);
-      if (x < p.nx) v = p.grid[outer * p.outer_stride + ai * p.axis_stride + x];
-      a[xi * p.m + i] = v;
+      if (j < p.xw.count) {
+        int x = p.xw.start + j;
+        x -= x >= p.nx ? p.nx : 0;
+        const long long ai = (long long)sub * p.sub_stride + (long long)i * p.estride;
+        if (!p.in_check || in_window(static_cast<int>(ai), p.in, p.nax))
+          v = base[ai * p.axis_stride + x];
+      }
+      a[xi * ld + i] = v;
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, p.m, TW, p.m, p.rad, threadIdx.x, blockDim.x);
+    cplx<R>* res = lds_fft<R>(a, b, p.m, TW, ld, p.rad, p.tw, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < TW * p.m; t += blockDim.x) {
       const int xi = t % TW, k = t / TW;
-      const long long x = x0 + xi;
-      if (x >= p.nx) continue;
-      cplx<R> v = res[xi * p.m + k];
-      if (p.twiddle) {
-        R sn, cs;
-        const long long e = ((long long)sub * k) % p.ntot;
-        if constexpr (sizeof(R) == 8)
-          sincospi(2.0 * (double)e / (double)p.ntot, &sn, &cs);
-        else
-          sincospif(2.0f * (float)e / (float)p.ntot, &sn, &cs);
-        v = v * mk<R>(cs, sn);
-      }
+      const int j = xt * TW + xi;
+      if (j >= p.xw.count) continue;
+      if (p.out_check && !in_window(sub * p.fa + k * p.fb, p.out, p.nax)) continue;
+      int x = p.xw.start + j;
+      x -= x >= p.nx ? p.nx : 0;
+      cplx<R> v = res[xi * ld + k];
+      if (p.twiddle) v = v * p.tw2[((long long)sub * k) % p.ntot];
       const long long ai = (long long)sub * p.sub_stride + (long long)k * p.estride;
-      p.grid[outer * p.outer_stride + ai * p.axis_stride + x] = v;
+      base[ai * p.axis_stride + x] = v;
     }
     __syncthreads();
   }
@@ -571,84 +636,110 @@ void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStr
 }
 
 void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const unsigned* bin_start, unsigned* bin_fill, unsigned* sorted,
-                       hipStream_t st) {
+                       const N3Point* pts, const unsigned* bin_start, unsigned* bin_fill,
+                       N3Point* sorted, hipStream_t st) {
   if (n_pts <= 0) return;
   n3_scatter_kernel<<<static_cast<unsigned>((n_pts + 255) / 256), 256, 0, st>>>(
-      n_pts, valid, bin_of, bin_start, bin_fill, sorted);
+      n_pts, valid, bin_of, pts, bin_start, bin_fill, sorted);
   check_launch("n3_scatter_kernel");
 }
 
-template <typename R, typename TV>
+template <typename R>
 size_t n3_spread_shmem(const N3SpreadParams<R>& p) {
-  const int tcells = (p.bin_size[0] + p.w) * (p.bin_size[1] + p.w) * (p.bin_size[2] + p.w);
-  return sizeof(R) * ((size_t)2 * p.n_levels * tcells + 32 * 3 * p.w + 32 * p.n_levels * 2) +
-         sizeof(int) * 32 * 3;
+  const size_t tcells = (size_t)p.bin_size[0] * p.bin_size[1] * p.bin_size[2];
+  return sizeof(R) * (2 * p.n_levels * tcells + SP_PB * 3 * p.w + SP_PB * p.n_levels * 2) +
+         sizeof(int) * (SP_PB * 8 + SP_PB + 1);
 }
 
 template <typename R, typename TV>
-void launch_n3_spread(const N3SpreadParams<R>& p, int n_bins, hipStream_t st) {
-  if (n_bins <= 0) return;
-  const size_t sh = n3_spread_shmem<R, TV>(p);
-  if (sh > 160 * 1024) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT spread tile exceeds LDS");
+void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
+  const long long n_blocks = (long long)p_all.nbins[0] * p_all.nbins[1] * p_all.nbins[2];
+  if (n_blocks <= 0) return;
+  if (n_blocks > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT spread blocks");
+  for (int d = 0; d < 3; ++d)
+    if ((long long)p_all.nbins[d] * p_all.bin_size[d] < p_all.nf1[d] || p_all.nback[d] < 0)
+      fail(BIPP_INTERNAL_ERROR, "NUFFT spread blocks do not cover the grid");
   static bool attr_set = false;
   if (!attr_set) {
     BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&n3_spread_kernel<R, TV>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
-  n3_spread_kernel<R, TV><<<n_bins, SP_NT, sh, st>>>(p);
-  check_launch("n3_spread_kernel");
+  // as many levels per launch as the LDS tile allows
+  const int L = p_all.n_levels_total;
+  int group = L;
+  for (; group > 1; --group) {
+    N3SpreadParams<R> t = p_all;
+    t.n_levels = group;
+    if (n3_spread_shmem<R>(t) <= 64 * 1024) break;
+  }
+  for (int lv0 = 0; lv0 < L; lv0 += group) {
+    N3SpreadParams<R> p = p_all;
+    p.lv0 = lv0;
+    p.n_levels = std::min(group, L - lv0);
+    const size_t sh = n3_spread_shmem<R>(p);
+    if (sh > 160 * 1024) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT spread tile exceeds LDS");
+    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), SP_NT, sh, st>>>(p);
+    check_launch("n3_spread_kernel");
+  }
 }
-template void launch_n3_spread<float, cplx<float>>(const N3SpreadParams<float>&, int, hipStream_t);
-template void launch_n3_spread<double, cplx<double>>(const N3SpreadParams<double>&, int,
-                                                     hipStream_t);
+template void launch_n3_spread<float, cplx<float>>(const N3SpreadParams<float>&, hipStream_t);
+template void launch_n3_spread<double, cplx<double>>(const N3SpreadParams<double>&, hipStream_t);
+
+namespace {
+bool window_ok(const N3Window& w, int n) {
+  return n > 0 && w.start >= 0 && w.start < n && w.count >= 0 && w.count <= n;
+}
+}  // namespace
 
 template <typename R>
-void launch_n3_deconv(const N3DeconvParams<R>& p, int n_levels, hipStream_t st) {
-  const long long n = (long long)p.nf1[0] * p.nf1[1] * p.nf1[2];
-  if (n <= 0 || n_levels <= 0) return;
-  dim3 grid(static_cast<unsigned>(std::min<long long>((n + 255) / 256, 16384)), n_levels);
-  n3_deconv_kernel<R><<<grid, 256, 0, st>>>(p);
-  check_launch("n3_deconv_kernel");
-}
-template void launch_n3_deconv<float>(const N3DeconvParams<float>&, int, hipStream_t);
-template void launch_n3_deconv<double>(const N3DeconvParams<double>&, int, hipStream_t);
-
-template <typename R>
-void launch_fft_x(cplx<R>* grid, int n, long long n_lines, const N3Radix& rad, hipStream_t st) {
+void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
+  const long long n_lines = (long long)p.n_levels * p.zw.count * p.yw.count;
   if (n_lines <= 0) return;
-  const size_t sh = 2 * sizeof(cplx<R>) * (size_t)n;
+  if (!window_ok(p.yw, p.ny) || !window_ok(p.zw, p.nz) || !window_ok(p.in, p.n) ||
+      !window_ok(p.out, p.n))
+    fail(BIPP_INTERNAL_ERROR, "FFT line windows out of range");
+  // several short lines per workgroup so that every stage keeps most lanes busy
+  const int nl = std::max(1, std::min(16, 2048 / p.n));
+  const size_t sh = 2 * sizeof(cplx<R>) * (size_t)p.n * nl;
+  // 1 KiB of the 160 KiB stays for the kernel's static line bases
+  if (sh > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
   static bool attr_set = false;
   if (!attr_set) {
-    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_x_kernel<R>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lines_kernel<R>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
     attr_set = true;
   }
-  const unsigned blocks = static_cast<unsigned>(std::min<long long>(n_lines, 8192));
-  fft_x_kernel<R><<<blocks, 256, sh, st>>>(grid, n, n_lines, rad);
-  check_launch("fft_x_kernel");
+  const long long groups = (n_lines + nl - 1) / nl;
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>(groups, 32768));
+  fft_lines_kernel<R><<<blocks, 256, sh, st>>>(p, nl);
+  check_launch("fft_lines_kernel");
 }
-template void launch_fft_x<float>(cplx<float>*, int, long long, const N3Radix&, hipStream_t);
-template void launch_fft_x<double>(cplx<double>*, int, long long, const N3Radix&, hipStream_t);
+template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
+template void launch_fft_lines<double>(const N3LineParams<double>&, hipStream_t);
 
 template <typename R>
-void launch_fft_strided(const N3StridedParams<R>& p, hipStream_t st) {
-  const long long items = ((p.nx + 15) / 16) * p.n_outer * p.n_sub;
+void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
+  const long long items =
+      (long long)((p.xw.count + 15) / 16) * p.n_sub * p.other.count * p.n_levels;
   if (items <= 0) return;
-  const size_t sh = 2 * 16 * sizeof(cplx<R>) * (size_t)p.m;
+  if (!window_ok(p.xw, p.nx) || !window_ok(p.other, p.no) || !window_ok(p.in, p.nax) ||
+      !window_ok(p.out, p.nax) || (long long)p.m * p.n_sub != p.nax)
+    fail(BIPP_INTERNAL_ERROR, "FFT pass windows out of range");
+  const size_t sh = 2 * 16 * sizeof(cplx<R>) * (size_t)(p.m | 1);
+  if (sh > 160 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");
   static bool attr_set = false;
   if (!attr_set) {
-    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_strided_kernel<R>),
+    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_pass_kernel<R>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
-  const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 16384));
-  fft_strided_kernel<R><<<blocks, 256, sh, st>>>(p);
-  check_launch("fft_strided_kernel");
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 32768));
+  fft_pass_kernel<R><<<blocks, 256, sh, st>>>(p);
+  check_launch("fft_pass_kernel");
 }
-template void launch_fft_strided<float>(const N3StridedParams<float>&, hipStream_t);
-template void launch_fft_strided<double>(const N3StridedParams<double>&, hipStream_t);
+template void launch_fft_pass<float>(const N3PassParams<float>&, hipStream_t);
+template void launch_fft_pass<double>(const N3PassParams<double>&, hipStream_t);
 
 template <typename R, typename TI>
 void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st) {

@@ -10,6 +10,17 @@ struct N3Radix {
   int r[20] = {};
 };
 
+// One non-uniform point of the spread (one antenna pair a >= b of one sample).
+struct alignas(16) N3Point {
+  int l0[3];    // first cell of the kernel support, spread index + nf1/2
+  int src;      // offset of VV(a, b) inside the sample's matrix
+  float fr[3];  // cell - (l0 - nf1/2) per axis, in [w/2, w/2 + 1)
+  float mult;   // 2 for a > b ((a,b) and (b,a) are conjugate), 1 for a == b
+  float ph[2];  // pre-phase exp(i u'.D)
+  int sample;
+  int pad;
+};
+
 struct N3PointsParams {
   int n_ant_max;                 // rows per sample (pairs a >= b < n_ant_max)
   const int* n_rows;             // active rows per sample, or nullptr -> n_rows_const
@@ -25,47 +36,73 @@ struct N3PointsParams {
   // outputs, [nS * npair]
   unsigned char* valid;
   int* bin_of;
-  int *l0x, *l0y, *l0z;
-  float *frx, *fry, *frz, *phr, *phi, *mult;
-  int* src;
+  N3Point* pts;
   unsigned* bin_count;
 };
 
+// Spread with output ownership: the workgroup of block b accumulates, in LDS, every point
+// of bins b - nback .. b whose kernel support meets the block, then writes its cells once,
+// deconvolved, with plain stores.
 template <typename R>
 struct N3SpreadParams {
-  int n_levels, w;
+  int n_levels, w;               // levels spread by this launch
+  int lv0, n_levels_total;       // first level of the group, levels stored per sample
   double beta;
-  int bin_size[3], nbins[3], off[3], nf2[3];
-  const unsigned *bin_start, *bin_count, *sorted;
-  const int *l0x, *l0y, *l0z, *src;
-  const float *frx, *fry, *frz, *phr, *phi, *mult;
-  long long npair;
+  int bin_size[3], nbins[3], nback[3];
+  int nf1[3], nf2[3];
+  const unsigned *bin_start, *bin_count;
+  const N3Point* pts;            // sorted by bin
   const cplx<R>* vv;             // [nS][L][vv_stride]
   long long vv_stride;
-  R* grid;                       // [L][grid_stride][2]
-  long long grid_stride;
-};
-
-template <typename R>
-struct N3DeconvParams {
-  int nf1[3], nf2[3];
   const R *fx, *fy, *fz;         // 1/phi2^(l) for l = -nf1/2 .. nf1/2-1
-  R* grid;
+  cplx<R>* grid;                 // [L][grid_stride]
   long long grid_stride;
 };
 
+// Window of an axis of length n: indices (start + i) mod n for i < count.
+struct N3Window {
+  int start, count;
+};
+
+// Contiguous-axis (x) transforms of the pruned FFT: lines (level, z in zw, y in yw); inputs
+// outside `in` are zero and are not read, outputs outside `out` are not written.
 template <typename R>
-struct N3StridedParams {
+struct N3LineParams {
   cplx<R>* grid;
-  long long nx;                  // contiguous extent tiled by 16
-  long long axis_stride;         // elements between consecutive axis indices
-  long long n_outer, outer_stride;
-  int m;                         // transform length
-  long long estride, sub_stride;
-  int n_sub;
-  int twiddle;
+  long long grid_stride;
+  int n_levels;
+  int n, ny, nz;
+  N3Window yw, zw, in, out;
+  N3Radix rad;
+  const cplx<R>* tw;             // exp(+2 pi i t / n), t < n
+};
+
+// Strided-axis transforms of the pruned FFT for x positions (x0 + j) mod nx, j < nxw, and
+// entries o < other.count of the remaining axis (storage index perm(start + o mod no)).
+// Element i of sub-transform `sub` sits at axis index sub * sub_stride + i * estride.
+template <typename R>
+struct N3PassParams {
+  cplx<R>* grid;
+  long long grid_stride;
+  int n_levels;
+  int nx;
+  N3Window xw;
+  int no;
+  N3Window other;
+  int operm_n1, operm_n2;        // operm_n2 > 1: storage (k mod n1) n2 + k / n1
+  long long outer_stride, axis_stride;
+  int m, n_sub;
+  long long sub_stride, estride;
+  int nax;                       // axis length
+  int in_check;                  // load only axis indices inside `in`
+  N3Window in;
+  int out_check, fa, fb;         // store output k only if frequency sub fa + k fb is in `out`
+  N3Window out;
+  int twiddle;                   // multiply output k of sub-transform sub by tw2[sub k mod ntot]
   long long ntot;
   N3Radix rad;
+  const cplx<R>* tw;             // exp(+2 pi i t / m), t < m
+  const cplx<R>* tw2;            // exp(+2 pi i t / ntot), t < ntot (when twiddle)
 };
 
 template <typename R>
@@ -106,16 +143,14 @@ template <typename R>
 void launch_n3_points(const N3PointsParams& p, int n_samples, hipStream_t st);
 void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStream_t st);
 void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const unsigned* bin_start, unsigned* bin_fill, unsigned* sorted,
-                       hipStream_t st);
+                       const N3Point* pts, const unsigned* bin_start, unsigned* bin_fill,
+                       N3Point* sorted, hipStream_t st);
 template <typename R, typename TV>
-void launch_n3_spread(const N3SpreadParams<R>& p, int n_bins, hipStream_t st);
+void launch_n3_spread(const N3SpreadParams<R>& p, hipStream_t st);
 template <typename R>
-void launch_n3_deconv(const N3DeconvParams<R>& p, int n_levels, hipStream_t st);
+void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st);
 template <typename R>
-void launch_fft_x(cplx<R>* grid, int n, long long n_lines, const N3Radix& rad, hipStream_t st);
-template <typename R>
-void launch_fft_strided(const N3StridedParams<R>& p, hipStream_t st);
+void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st);
 template <typename R, typename TI>
 void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st);
 template <typename R, typename TO>

@@ -136,10 +136,12 @@ BippError bipp_synthesis_reset(BippSynthesis h);
 /* Eigenvalues (float64, nBeam per sample, ascending, zero padded) and kept-beam counts of
  * the samples of the last collect call, host arrays. */
 BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, double* d, int32_t* nBeamReduced);
-/* GPU time per stage accumulated over collect calls, from HIP events on the context stream:
- * ms[0] eigensolve (flag, Gram, solve), ms[1] level weights / operand preparation,
- * ms[2] imaging kernels, ms[3] number of collect batches timed.  The reference's analogue is
- * its wall-clock timer tree (src/rt_graph.hpp:94-161).  reset != 0 clears the counters. */
+/* GPU time per stage accumulated over collect calls, from HIP events on the context stream
+ * (ms must hold 9 doubles): ms[0] eigensolve (flag, Gram, solve), ms[1] level weights /
+ * operand preparation, ms[2] imaging kernels, ms[3] number of collect batches timed; for the
+ * NUFFT imager ms[4] virtual visibilities, ms[5] point binning and sort, ms[6] spreading,
+ * ms[7] deconvolution + FFT, ms[8] interpolation.  The reference's analogue is its
+ * wall-clock timer tree (src/rt_graph.hpp:94-161).  reset != 0 clears the counters. */
 BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset);
 /* NUFFT tolerance of imager 2 (NufftSynthesisOptions::tolerance,
  * include/bipp/image_synthesis.hpp:61-65); default 1e-5. */
