@@ -11,6 +11,8 @@
 #include <vector>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "bipp_hip.h"
 #include "common.hpp"
@@ -133,7 +135,7 @@ struct ContextImpl {
 // ---------------------------------------------------------------------------------------
 template <typename R>
 struct EigRunner {
-  DevBuf n_red, idx, n_vis, info, counter, order, d, z, ws_a, ws_l, ws_z;
+  DevBuf n_red, idx, n_vis, info, counter, order, d, z, ws_a, ws_l, ws_z, prof_buf;
   DevBuf sig, w_i, tmp, g, v_full;
   int n_samples = 0, n_ant = 0, nmax = 0;
   bool general_w = false;
@@ -272,7 +274,33 @@ struct EigRunner {
     ep.ws_l = ws_l.as<cplx<R>>((size_t)n_slots * nn);
     ep.ws_z = ws_z.as<R>((size_t)n_slots * nn);
     ep.info = info.as<int>(nS);
+    // BIPP_EIG_PROFILE=1: per-phase wall-clock averages of this launch on stderr
+    const bool prof = std::getenv("BIPP_EIG_PROFILE") != nullptr;
+    if (prof) {
+      ep.prof = prof_buf.as<unsigned long long>((size_t)nS * 8);
+      BIPP_HIP(hipMemsetAsync(ep.prof, 0, sizeof(unsigned long long) * nS * 8, st));
+    }
     launch_eig<R>(ep, n_slots, s_double, st);
+    if (prof) {
+      std::vector<unsigned long long> h((size_t)nS * 8);
+      BIPP_HIP(hipMemcpyAsync(h.data(), ep.prof, sizeof(unsigned long long) * h.size(),
+                              hipMemcpyDeviceToHost, st));
+      BIPP_HIP(hipStreamSynchronize(st));
+      static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "ql",
+                                     "backtransform", "unwhiten"};
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+      int cnt = 0;
+      for (int k = 0; k < nS; ++k) {
+        const unsigned long long* r = &h[(size_t)k * 8];
+        if (!r[0] || !r[7]) continue;
+        ++cnt;
+        for (int q = 0; q < 7; ++q) acc[q] += r[q + 1] >= r[q] && r[q] ? double(r[q + 1] - r[q]) : 0.0;
+      }
+      std::fprintf(stderr, "[bipp eig profile] %d samples, n_slots %d, us/sample (100 MHz clock):", cnt,
+                   n_slots);
+      for (int q = 0; q < 7; ++q) std::fprintf(stderr, " %s %.1f", names[q], cnt ? acc[q] / cnt / 100.0 : 0.0);
+      std::fprintf(stderr, "\n");
+    }
 
     if (general_w) {
       // V = W_I Z (src/eigensolver.cpp:214-216)
@@ -533,12 +561,50 @@ struct N3Plan {
   long long n_bins() const { return (long long)nbins[0] * nbins[1] * nbins[2]; }
 };
 
+// Summary of a profiled spread launch: block durations (100 MHz wall clock), visited points
+// and items, and the slowest blocks with their bin coordinates.
+inline void report_spread_profile(const unsigned long long* dprof, long long nb, const N3Plan& pl,
+                                  hipStream_t st) {
+  std::vector<unsigned long long> h((size_t)nb * 4);
+  BIPP_HIP(hipMemcpyAsync(h.data(), dprof, sizeof(unsigned long long) * h.size(),
+                          hipMemcpyDeviceToHost, st));
+  BIPP_HIP(hipStreamSynchronize(st));
+  unsigned long long t0 = ~0ull, t1 = 0;
+  double sum_dur = 0, sum_pts = 0, sum_items = 0;
+  std::vector<std::pair<double, long long>> dur;
+  dur.reserve(nb);
+  for (long long b = 0; b < nb; ++b) {
+    const unsigned long long* r = &h[(size_t)b * 4];
+    if (!r[0]) continue;
+    t0 = std::min(t0, r[0]);
+    t1 = std::max(t1, r[1]);
+    const double d = double(r[1] - r[0]) / 100.0;
+    sum_dur += d;
+    sum_pts += double(r[2]);
+    sum_items += double(r[3]);
+    dur.emplace_back(d, b);
+  }
+  std::sort(dur.begin(), dur.end(), [](auto& a, auto& b) { return a.first > b.first; });
+  std::fprintf(stderr,
+               "[bipp spread profile] blocks %lld span %.1f us, sum of block us %.0f, visits %.3g, "
+               "items %.3g, median block %.1f us\n",
+               (long long)dur.size(), double(t1 - t0) / 100.0, sum_dur, sum_pts, sum_items,
+               dur.empty() ? 0.0 : dur[dur.size() / 2].first);
+  for (size_t i = 0; i < std::min<size_t>(5, dur.size()); ++i) {
+    const long long b = dur[i].second;
+    const unsigned long long* r = &h[(size_t)b * 4];
+    std::fprintf(stderr, "  block (%lld,%lld,%lld) %.1f us, visits %llu, items %llu\n",
+                 b % pl.nbins[0], (b / pl.nbins[0]) % pl.nbins[1], b / ((long long)pl.nbins[0] * pl.nbins[1]),
+                 dur[i].first, r[2], r[3]);
+  }
+}
+
 // Runs the NUFFT image stage for one batch: points from positions, values from
 // VV_l = Z diag(w_l) Z^H, all levels accumulated into `img` (TI per pixel).
 template <typename TG>
 struct N3Runner {
   DevBuf grid, valid, bin_of, pts, sorted;
-  DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid;
+  DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid, prof_buf;
   std::vector<cplx<TG>> htw;
   std::vector<TG> hdec;
 
@@ -569,6 +635,8 @@ struct N3Runner {
     vp.vv_stride = vv_stride;
     vp.ld_vv = ldv;
     launch_vv_levels<RE, TG>(vp, nS, rows_max, st);
+    const bool merge_auto = rows_max >= 2;
+    if (merge_auto) launch_vv_trace<TG>(nS, L, n_rows, rows_max, vvp, vv_stride, ldv, st);
     mark(1);
 
     // points, bins, sort
@@ -593,6 +661,7 @@ struct N3Runner {
     pp.w = pl.w;
     pp.phase_fp32 = phase_fp32 ? 1 : 0;
     pp.ld_vv = ldv;
+    pp.merge_auto = merge_auto ? 1 : 0;
     pp.valid = valid.as<unsigned char>(npts);
     pp.bin_of = bin_of.as<int>(npts);
     pp.pts = pts.as<N3Point>(npts);
@@ -644,7 +713,12 @@ struct N3Runner {
     sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
     sp.grid = cg;
     sp.grid_stride = G;
+    // BIPP_N3_PROFILE=1: per-block duration / work summary of the spread on stderr
+    const bool prof = std::getenv("BIPP_N3_PROFILE") != nullptr;
+    sp.prof = prof ? prof_buf.as<unsigned long long>((size_t)nb * 4) : nullptr;
+    if (prof) BIPP_HIP(hipMemsetAsync(sp.prof, 0, sizeof(unsigned long long) * nb * 4, st));
     launch_n3_spread<TG, cplx<TG>>(sp, st);
+    if (prof) report_spread_profile(sp.prof, nb, pl, st);
     mark(3);
 
     // pruned FFT (sign +) along x, y, z; twiddle tables exp(+2 pi i t / n) computed in FP64
@@ -652,12 +726,15 @@ struct N3Runner {
     // table segments: x, then per strided axis n1, n2 and n1 * n2
     size_t tw_off[3][3] = {};
     htw.clear();
+    // two-level tables: exp(+2 pi i t / n) = fine[t mod 64] * coarse[t / 64]
     auto add_table = [&](long long n) {
       const size_t o = htw.size();
-      for (long long t = 0; t < n; ++t) {
-        const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t) / n;
-        htw.push_back(mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang))));
-      }
+      auto root = [&](long long t) {
+        const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t % n) / n;
+        return mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang)));
+      };
+      for (long long t = 0; t < 64; ++t) htw.push_back(root(t));
+      for (long long s = 0; s < (n + 63) / 64; ++s) htw.push_back(root(64 * s));
       return o;
     };
     tw_off[0][0] = add_table(nx);

@@ -21,8 +21,20 @@ namespace bippamd {
 
 namespace {
 
+// Threads per workgroup for the small kernels of this file.
 constexpr int NT = 256;
 constexpr int NW = NT / 64;
+
+// Eigensolver launch shape per size bucket: wider workgroups for the larger problems, so
+// that more columns are in flight per step; the per-sample matrices live in global memory
+// (HBM at these sizes), so every step is bound by the number of dependent round trips.
+template <int NMAX>
+struct EigCfg {
+  static constexpr int NT = NMAX >= 256 ? 512 : 256;
+  static constexpr int NW = NT / 64;
+  static constexpr int RM = NMAX / 64;          // rows per lane for a full column
+  static constexpr int UC = RM >= 8 ? 2 : 4;    // columns per wave loaded together
+};
 
 template <typename R>
 __device__ __forceinline__ R sinpi_r(R x);
@@ -51,23 +63,70 @@ __device__ __forceinline__ cplx<R> crecip(cplx<R> a) {
   }
 }
 
+// State of the tridiagonal QL iteration, advanced by one generator thread.
+template <typename R>
+struct QlState {
+  R f, tst1;
+  int l, m, iter, in_iter, fail;
+};
+
 template <typename R, int NMAX>
 struct EigShared {
+  static constexpr int NW_ = EigCfg<NMAX>::NW;
   cplx<R> v[NMAX];
   cplx<R> w[NMAX];
   cplx<R> vn[NMAX];
-  cplx<R> xw[NW][NMAX];
+  cplx<R> xw[NW_][NMAX];
   R dd[NMAX];
   R ee[NMAX];
-  R cs[NMAX];
-  R sn[NMAX];
+  R cs[2][NMAX];
+  R sn[2][NMAX];
   int perm[NMAX];
   int idx[NMAX];
-  R red[2 * NW];
+  R red[2 * NW_];
   R scal[4];
   int iscal[4];
-  cplx<R> ctau;
+  int sweep[2][3];  // (l, m, valid) of the rotation buffers
+  QlState<R> ql;
 };
+
+// --------------------------------------------------------------------------------------
+// B[i, j] -= x[i] * y_j for columns j = j0 + wave + t NW < n and rows i in [rlo(j), rhi):
+// the wave loads UC columns (RM rows per lane each) before it stores any of them.
+// y_j = y[j], or conj(y[j]) with CONJY.
+// --------------------------------------------------------------------------------------
+template <typename R, int NMAX, bool CONJY, typename RowLo>
+__device__ __forceinline__ void cols_update(int j0, int n, int rhi, RowLo rlo,
+                                            cplx<R>* __restrict__ B, int ldb, const cplx<R>* x,
+                                            const cplx<R>* y) {
+  constexpr int NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM, UC = EigCfg<NMAX>::UC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int jb = j0 + wave; jb < n; jb += NW_ * UC) {
+    cplx<R> val[UC][RM];
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {
+      const int j = jb + u * NW_;
+      const int lo = j < n ? rlo(j) : rhi;
+#pragma unroll
+      for (int r = 0; r < RM; ++r) {
+        const int i = lane + 64 * r;
+        if (i >= lo && i < rhi) val[u][r] = B[i + (size_t)j * ldb];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {
+      const int j = jb + u * NW_;
+      if (j >= n) continue;
+      const int lo = rlo(j);
+      const cplx<R> yj = CONJY ? conj(y[j]) : y[j];
+#pragma unroll
+      for (int r = 0; r < RM; ++r) {
+        const int i = lane + 64 * r;
+        if (i >= lo && i < rhi) B[i + (size_t)j * ldb] = val[u][r] - x[i] * yj;
+      }
+    }
+  }
+}
 
 // --------------------------------------------------------------------------------------
 // Forward substitution B <- L^-1 B on all n columns of B (column-major, ld).
@@ -75,21 +134,18 @@ struct EigShared {
 template <typename R, int NMAX>
 __device__ void fwd_solve(int n, const cplx<R>* __restrict__ L, int ldl, cplx<R>* __restrict__ B,
                           int ldb, EigShared<R, NMAX>& sh) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NT_ = EigCfg<NMAX>::NT;
+  const int tid = threadIdx.x;
   for (int k = 0; k < n; ++k) {
     const R linv = R(1) / L[k + (size_t)k * ldl].re;
-    for (int j = tid; j < n; j += NT) {
+    for (int j = tid; j < n; j += NT_) {
       cplx<R> r = B[k + (size_t)j * ldb] * linv;
       B[k + (size_t)j * ldb] = r;
       sh.w[j] = r;
     }
-    for (int i = k + 1 + tid; i < n; i += NT) sh.v[i] = L[i + (size_t)k * ldl];
+    for (int i = k + 1 + tid; i < n; i += NT_) sh.v[i] = L[i + (size_t)k * ldl];
     __syncthreads();
-    for (int j = wave; j < n; j += NW) {
-      const cplx<R> rj = sh.w[j];
-      cplx<R>* col = B + (size_t)j * ldb;
-      for (int i = k + 1 + lane; i < n; i += 64) col[i] = col[i] - sh.v[i] * rj;
-    }
+    cols_update<R, NMAX, false>(0, n, n, [k](int) { return k + 1; }, B, ldb, sh.v, sh.w);
     __syncthreads();
   }
 }
@@ -98,21 +154,18 @@ __device__ void fwd_solve(int n, const cplx<R>* __restrict__ L, int ldl, cplx<R>
 template <typename R, int NMAX>
 __device__ void bwd_solve_h(int n, const cplx<R>* __restrict__ L, int ldl, cplx<R>* __restrict__ B,
                             int ldb, EigShared<R, NMAX>& sh) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NT_ = EigCfg<NMAX>::NT;
+  const int tid = threadIdx.x;
   for (int k = n - 1; k >= 0; --k) {
     const R linv = R(1) / L[k + (size_t)k * ldl].re;
-    for (int j = tid; j < n; j += NT) {
+    for (int j = tid; j < n; j += NT_) {
       cplx<R> r = B[k + (size_t)j * ldb] * linv;
       B[k + (size_t)j * ldb] = r;
       sh.w[j] = r;
     }
-    for (int i = tid; i < k; i += NT) sh.v[i] = conj(L[k + (size_t)i * ldl]);
+    for (int i = tid; i < k; i += NT_) sh.v[i] = conj(L[k + (size_t)i * ldl]);
     __syncthreads();
-    for (int j = wave; j < n; j += NW) {
-      const cplx<R> rj = sh.w[j];
-      cplx<R>* col = B + (size_t)j * ldb;
-      for (int i = lane; i < k; i += 64) col[i] = col[i] - sh.v[i] * rj;
-    }
+    cols_update<R, NMAX, false>(0, n, k, [](int) { return 0; }, B, ldb, sh.v, sh.w);
     __syncthreads();
   }
 }
@@ -122,12 +175,13 @@ __device__ void bwd_solve_h(int n, const cplx<R>* __restrict__ L, int ldl, cplx<
 template <typename R, int NMAX>
 __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx<R>* vec,
                              EigShared<R, NMAX>& sh) {
+  constexpr int NT_ = EigCfg<NMAX>::NT;
   const int tid = threadIdx.x;
   cplx<R>* col = A + (size_t)c * ld;
   const cplx<R> alpha = col[c + 1];
   R part = 0;
-  for (int i = c + 2 + tid; i < n; i += NT) part += abs2(col[i]);
-  const R xnorm2 = block_sum<NT>(part, sh.red);
+  for (int i = c + 2 + tid; i < n; i += NT_) part += abs2(col[i]);
+  const R xnorm2 = block_sum<NT_>(part, sh.red);
   cplx<R> tau, scal;
   R beta;
   if (xnorm2 == R(0) && alpha.im == R(0)) {
@@ -140,7 +194,7 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
     tau = mk<R>((beta - alpha.re) / beta, -alpha.im / beta);
     scal = crecip(mk<R>(alpha.re - beta, alpha.im));
   }
-  for (int i = c + 2 + tid; i < n; i += NT) {
+  for (int i = c + 2 + tid; i < n; i += NT_) {
     cplx<R> vi = col[i] * scal;
     col[i] = vi;
     vec[i] = vi;
@@ -160,47 +214,64 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
 template <typename R, int NMAX, bool APPLY>
 __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const cplx<R>* v,
                            const cplx<R>* vn, EigShared<R, NMAX>& sh) {
-  constexpr int RMAX = NMAX / 64;
+  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
+                UC = EigCfg<NMAX>::UC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  cplx<R> acc[RMAX], vi[RMAX], wi[RMAX];
+  cplx<R> acc[RM], vi[RM], wi[RM];
 #pragma unroll
-  for (int r = 0; r < RMAX; ++r) {
+  for (int r = 0; r < RM; ++r) {
     acc[r] = mk<R>(0, 0);
     const int i = i0 + lane + 64 * r;
     vi[r] = (APPLY && i < n) ? v[i] : mk<R>(0, 0);
     wi[r] = (APPLY && i < n) ? sh.w[i] : mk<R>(0, 0);
   }
-  for (int j = i0 + wave; j < n; j += NW) {
-    cplx<R> cvj = mk<R>(0, 0), cwj = mk<R>(0, 0);
-    if (APPLY) {
-      cvj = conj(v[j]);
-      cwj = conj(sh.w[j]);
-    }
-    const cplx<R> vnj = vn[j];
-    cplx<R>* col = A + (size_t)j * ld;
+  for (int jb = i0 + wave; jb < n; jb += NW_ * UC) {
+    cplx<R> a[UC][RM];
 #pragma unroll
-    for (int r = 0; r < RMAX; ++r) {
-      const int i = i0 + lane + 64 * r;
-      if (i < n) {
-        cplx<R> a = col[i];
-        if (APPLY) {
-          a = a - (vi[r] * cwj + wi[r] * cvj);
-          col[i] = a;
+    for (int u = 0; u < UC; ++u) {
+      const int j = jb + u * NW_;
+      const cplx<R>* col = A + (size_t)j * ld;
+#pragma unroll
+      for (int r = 0; r < RM; ++r) {
+        const int i = i0 + lane + 64 * r;
+        if (j < n && i < n) a[u][r] = col[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {
+      const int j = jb + u * NW_;
+      if (j >= n) continue;
+      cplx<R> cvj = mk<R>(0, 0), cwj = mk<R>(0, 0);
+      if (APPLY) {
+        cvj = conj(v[j]);
+        cwj = conj(sh.w[j]);
+      }
+      const cplx<R> vnj = vn[j];
+      cplx<R>* col = A + (size_t)j * ld;
+#pragma unroll
+      for (int r = 0; r < RM; ++r) {
+        const int i = i0 + lane + 64 * r;
+        if (i < n) {
+          cplx<R> x = a[u][r];
+          if (APPLY) {
+            x = x - (vi[r] * cwj + wi[r] * cvj);
+            col[i] = x;
+          }
+          acc[r] = acc[r] + x * vnj;
         }
-        acc[r] = acc[r] + a * vnj;
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < RMAX; ++r) {
+  for (int r = 0; r < RM; ++r) {
     const int i = i0 + lane + 64 * r;
     if (i < n) sh.xw[wave][i] = acc[r];
   }
   __syncthreads();
-  for (int i = i0 + tid; i < n; i += NT) {
+  for (int i = i0 + tid; i < n; i += NT_) {
     cplx<R> s = sh.xw[0][i];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) s = s + sh.xw[w][i];
+    for (int w = 1; w < NW_; ++w) s = s + sh.xw[w][i];
     sh.w[i] = s;
   }
   __syncthreads();
@@ -210,30 +281,126 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
 // On entry sh.w holds x (un-scaled); on exit sh.w holds w.
 template <typename R, int NMAX>
 __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<R, NMAX>& sh) {
+  constexpr int NT_ = EigCfg<NMAX>::NT;
   const int tid = threadIdx.x;
   R pr = 0, pi = 0;
-  for (int i = c + 1 + tid; i < n; i += NT) {
+  for (int i = c + 1 + tid; i < n; i += NT_) {
     cplx<R> x = tau * sh.w[i];
     sh.w[i] = x;
     cplx<R> t = cmul_conj_a(x, vec[i]);  // conj(x) v
     pr += t.re;
     pi += t.im;
   }
-  const R dr = block_sum<NT>(pr, sh.red);
-  const R di = block_sum<NT>(pi, sh.red);
+  const R dr = block_sum<NT_>(pr, sh.red);
+  const R di = block_sum<NT_>(pi, sh.red);
   const cplx<R> alpha2 = tau * mk<R>(R(-0.5) * dr, R(-0.5) * di);
-  for (int i = c + 1 + tid; i < n; i += NT) sh.w[i] = sh.w[i] + alpha2 * vec[i];
+  for (int i = c + 1 + tid; i < n; i += NT_) sh.w[i] = sh.w[i] + alpha2 * vec[i];
   __syncthreads();
+}
+
+// Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
+// next sweep and stores that sweep's rotations i = m-1 .. l in cs/sn[buf]; returns false
+// when every eigenvalue has converged (or the iteration limit was hit, ql.fail = 1).
+// Runs on one thread; d, e live in LDS.
+template <typename R, int NMAX>
+__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf) {
+  QlState<R>& q = sh.ql;
+  R* d = sh.dd;
+  R* e = sh.ee;
+  const R eps = eps_r<R>();
+  while (q.l < n) {
+    const int l = q.l;
+    if (!q.in_iter) {
+      q.tst1 = fmax(q.tst1, fabs(d[l]) + fabs(e[l]));
+      int m = l;
+      while (m < n - 1 && fabs(e[m]) > eps * q.tst1) ++m;
+      if (m == l) {
+        d[l] += q.f;
+        e[l] = 0;
+        ++q.l;
+        continue;
+      }
+      q.m = m;
+      q.in_iter = 1;
+      q.iter = 0;
+    }
+    if (++q.iter > 30 * n) {
+      q.fail = 1;
+      q.l = n;
+      return false;
+    }
+    const int m = q.m;
+    R g = d[l];
+    R pp = (d[l + 1] - g) / (R(2) * e[l]);
+    R r = sqrt(pp * pp + R(1));
+    if (pp < 0) r = -r;
+    d[l] = e[l] / (pp + r);
+    d[l + 1] = e[l] * (pp + r);
+    const R dl1 = d[l + 1];
+    R h = g - d[l];
+    for (int i = l + 2; i < n; ++i) d[i] -= h;
+    q.f += h;
+    pp = d[m];
+    R c = 1, c2 = 1, c3 = 1, s = 0, s2 = 0;
+    const R el1 = e[l + 1];
+    R* cs = sh.cs[buf];
+    R* sn = sh.sn[buf];
+    for (int i = m - 1; i >= l; --i) {
+      c3 = c2;
+      c2 = c;
+      s2 = s;
+      g = c * e[i];
+      h = c * pp;
+      // Givens rotation zeroing e[i]: one square root and one reciprocal
+      const R ei = e[i];
+      r = sqrt(pp * pp + ei * ei);
+      e[i + 1] = s * r;
+      if (r != R(0)) {
+        const R rinv = R(1) / r;
+        s = ei * rinv;
+        c = pp * rinv;
+      } else {
+        s = R(0);
+        c = R(1);
+      }
+      pp = c * d[i] - s * g;
+      d[i + 1] = h + s * (c * g + s * d[i]);
+      cs[i] = c;
+      sn[i] = s;
+    }
+    pp = -s * s2 * c3 * el1 * e[l] / dl1;
+    e[l] = s * pp;
+    d[l] = c * pp;
+    sh.sweep[buf][0] = l;
+    sh.sweep[buf][1] = m;
+    if (!(fabs(e[l]) > eps * q.tst1)) {
+      // converged: this sweep is the last one for l
+      d[l] += q.f;
+      e[l] = 0;
+      ++q.l;
+      q.in_iter = 0;
+    }
+    return true;
+  }
+  return false;
+}
+
+// records the wall clock at phase boundaries when the caller asked for a profile
+template <typename R>
+__device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
+  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 8 + k] = wall_clock64();
 }
 
 template <typename R, typename TIn, int NMAX>
 __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh) {
+  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
+                UC = EigCfg<NMAX>::UC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = p.n_red[sid];
   const int nmax = p.nmax;
   R* dout = p.d_out + (size_t)sid * p.stride_d;
   if (n <= 0) {
-    for (int j = tid; j < nmax; j += NT) dout[j] = 0;
+    for (int j = tid; j < nmax; j += NT_) dout[j] = 0;
     if (tid == 0) p.info[sid] = 0;
     return;
   }
@@ -245,8 +412,9 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   cplx<R>* X = p.z_out + (size_t)sid * p.stride_z;
   const int ldx = p.ldz;
 
-  for (int i = tid; i < n; i += NT) sh.idx[i] = p.idx[(size_t)sid * p.idx_stride + i];
+  for (int i = tid; i < n; i += NT_) sh.idx[i] = p.idx[(size_t)sid * p.idx_stride + i];
   __syncthreads();
+  eig_mark(p, sid, 0);
 
   // ---- load the reduced pencil -------------------------------------------------------
   {
@@ -255,7 +423,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     const R wl = static_cast<R>(p.wl[sid]);
     const R pref = static_cast<R>(p.gram_prefactor);
     const cplx<R>* Gin = p.g_in ? p.g_in + (size_t)sid * p.stride_g : nullptr;
-    for (int j = wave; j < n; j += NW) {
+    for (int j = wave; j < n; j += NW_) {
       const int bj = sh.idx[j];
       for (int i = lane; i < n; i += 64) {
         const int bi = sh.idx[i];
@@ -284,47 +452,41 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     }
     __syncthreads();
   }
+  eig_mark(p, sid, 1);
 
   if (!p.standard) {
     // ---- Cholesky G = L L^H (lower, right-looking) -------------------------------------
     for (int j = 0; j < n; ++j) {
-      if (tid == 0) {
-        const R djj = L[j + (size_t)j * ld].re;
-        const bool ok = djj > R(0) && djj == djj;
-        const R sj = ok ? sqrt(djj) : R(1);
-        L[j + (size_t)j * ld] = mk<R>(sj, 0);
-        sh.scal[0] = R(1) / sj;
-        sh.iscal[0] = ok;
-      }
-      __syncthreads();
-      if (!sh.iscal[0]) {
+      // every thread reads the pivot (one broadcast load per wave): no barrier needed
+      const R djj = L[j + (size_t)j * ld].re;
+      const bool ok = djj > R(0) && djj == djj;
+      if (!ok) {
         // leading minor j+1 of G is not positive definite: ?hegv reports info = n + j + 1
-        for (int jj = tid; jj < nmax; jj += NT) dout[jj] = 0;
+        for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
         if (tid == 0) p.info[sid] = n + j + 1;
         __syncthreads();
         return;
       }
-      const R inv = sh.scal[0];
-      for (int i = j + 1 + tid; i < n; i += NT) {
+      const R sj = sqrt(djj);
+      const R inv = R(1) / sj;
+      for (int i = j + 1 + tid; i < n; i += NT_) {
         cplx<R> u = L[i + (size_t)j * ld] * inv;
         L[i + (size_t)j * ld] = u;
         sh.v[i] = u;
       }
-      __syncthreads();
-      for (int k = j + 1 + wave; k < n; k += NW) {
-        const cplx<R> ck = conj(sh.v[k]);
-        cplx<R>* col = L + (size_t)k * ld;
-        for (int i = k + lane; i < n; i += 64) col[i] = col[i] - sh.v[i] * ck;
-      }
+      __syncthreads();  // every thread has read the pivot; sh.v complete
+      if (tid == 0) L[j + (size_t)j * ld] = mk<R>(sj, 0);
+      cols_update<R, NMAX, true>(j + 1, n, n, [](int kk) { return kk; }, L, ld, sh.v, sh.v);
       __syncthreads();
     }
+  eig_mark(p, sid, 2);
     // ---- C = L^-1 S L^-H ---------------------------------------------------------------
     fwd_solve<R, NMAX>(n, L, ld, A, ld, sh);  // A = L^-1 S
-    for (int j = wave; j < n; j += NW)
+    for (int j = wave; j < n; j += NW_)
       for (int i = lane; i < n; i += 64) X[i + (size_t)j * ldx] = conj(A[j + (size_t)i * ld]);
     __syncthreads();
     fwd_solve<R, NMAX>(n, L, ld, X, ldx, sh);  // X = L^-1 (L^-1 S)^H = C
-    for (int j = wave; j < n; j += NW)
+    for (int j = wave; j < n; j += NW_)
       for (int i = lane; i < n; i += 64) {
         cplx<R> c = (i >= j) ? X[i + (size_t)j * ldx] : conj(X[j + (size_t)i * ldx]);
         if (i == j) c.im = 0;
@@ -332,6 +494,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
       }
     __syncthreads();
   }
+  eig_mark(p, sid, 3);
 
   // ---- Householder tridiagonalisation (zhetd2, lower, fused her2 + hemv) --------------
   if (n == 1) {
@@ -356,7 +519,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
         const cplx<R> cw = conj(sh.w[k + 1]);
         const cplx<R> cv = conj(vcur[k + 1]);
         cplx<R>* col = A + (size_t)(k + 1) * ld;
-        for (int i = k + 1 + tid; i < n; i += NT) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
+        for (int i = k + 1 + tid; i < n; i += NT_) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
       }
       __syncthreads();
       if (tid == 0) sh.dd[k + 1] = A[(k + 1) + (size_t)(k + 1) * ld].re;
@@ -378,122 +541,68 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     }
     __syncthreads();
   }
+  eig_mark(p, sid, 4);
 
   // ---- implicit QL on the real tridiagonal (tql2), rotations into Z -------------------
-  for (int j = 0; j < n; ++j)
-    for (int i = tid; i < n; i += NT) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
-  if (tid == 0) {
-    sh.scal[1] = 0;  // f
-    sh.scal[2] = 0;  // tst1
-    sh.iscal[2] = 0; // failure flag
+  // A generator thread produces the rotations of sweep t+1 while the row owners apply
+  // those of sweep t (double-buffered); the generator is a thread without a row of Z when
+  // the workgroup has one.
+  for (int j = wave; j < n; j += NW_)
+    for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
+  const int gen = n <= NT_ - 64 ? NT_ - 64 : 0;
+  if (tid == gen) {
+    sh.ql.f = 0;
+    sh.ql.tst1 = 0;
+    sh.ql.l = 0;
+    sh.ql.m = 0;
+    sh.ql.iter = 0;
+    sh.ql.in_iter = 0;
+    sh.ql.fail = 0;
+    sh.sweep[0][2] = ql_next_sweep<R, NMAX>(n, sh, 0) ? 1 : 0;
   }
   __syncthreads();
-  const R eps = eps_r<R>();
-  for (int l = 0; l < n; ++l) {
-    if (tid == 0) {
-      R tst1 = fmax(sh.scal[2], fabs(sh.dd[l]) + fabs(sh.ee[l]));
-      sh.scal[2] = tst1;
-      int m = l;
-      while (m < n - 1 && fabs(sh.ee[m]) > eps * tst1) ++m;
-      sh.iscal[1] = m;
-    }
-    __syncthreads();
-    const int m = sh.iscal[1];
-    if (m > l) {
-      int iter = 0;
-      for (;;) {
-        if (tid == 0) {
-          R* d = sh.dd;
-          R* e = sh.ee;
-          R g = d[l];
-          R pp = (d[l + 1] - g) / (R(2) * e[l]);
-          R r = sqrt(pp * pp + R(1));
-          if (pp < 0) r = -r;
-          d[l] = e[l] / (pp + r);
-          d[l + 1] = e[l] * (pp + r);
-          const R dl1 = d[l + 1];
-          R h = g - d[l];
-          for (int i = l + 2; i < n; ++i) d[i] -= h;
-          sh.scal[1] += h;
-          pp = d[m];
-          R c = 1, c2 = 1, c3 = 1, s = 0, s2 = 0;
-          const R el1 = e[l + 1];
-          for (int i = m - 1; i >= l; --i) {
-            c3 = c2;
-            c2 = c;
-            s2 = s;
-            g = c * e[i];
-            h = c * pp;
-            // Givens rotation zeroing e[i]: one square root and one reciprocal
-            const R ei = e[i];
-            r = sqrt(pp * pp + ei * ei);
-            e[i + 1] = s * r;
-            if (r != R(0)) {
-              const R rinv = R(1) / r;
-              s = ei * rinv;
-              c = pp * rinv;
-            } else {
-              s = R(0);
-              c = R(1);
-            }
-            pp = c * d[i] - s * g;
-            d[i + 1] = h + s * (c * g + s * d[i]);
-            sh.cs[i] = c;
-            sh.sn[i] = s;
-          }
-          pp = -s * s2 * c3 * el1 * e[l] / dl1;
-          e[l] = s * pp;
-          d[l] = c * pp;
-          sh.iscal[3] = !(fabs(e[l]) > eps * sh.scal[2]);
-        }
-        __syncthreads();
-        // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
-        // fetched 16 at a time so that their loads are in flight together
-        constexpr int CH = 16;
-        for (int row = tid; row < n; row += NT) {
-          R carry = Z[row + (size_t)m * ld];
-          for (int i0 = m - 1; i0 >= l; i0 -= CH) {
-            const int cnt = min(CH, i0 - l + 1);
-            R zc[CH];
+  for (int t = 0;; ++t) {
+    const int cur = t & 1;
+    if (!sh.sweep[cur][2]) break;
+    const int l = sh.sweep[cur][0], m = sh.sweep[cur][1];
+    if (tid == gen) sh.sweep[cur ^ 1][2] = ql_next_sweep<R, NMAX>(n, sh, cur ^ 1) ? 1 : 0;
+    // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
+    // fetched CH at a time so that their loads are in flight together
+    constexpr int CH = 32;
+    const R* cs = sh.cs[cur];
+    const R* sn = sh.sn[cur];
+    for (int row = tid; row < n; row += NT_) {
+      R carry = Z[row + (size_t)m * ld];
+      for (int i0 = m - 1; i0 >= l; i0 -= CH) {
+        const int cnt = min(CH, i0 - l + 1);
+        R zc[CH];
 #pragma unroll
-            for (int t = 0; t < CH; ++t)
-              if (t < cnt) zc[t] = Z[row + (size_t)(i0 - t) * ld];
+        for (int u = 0; u < CH; ++u)
+          if (u < cnt) zc[u] = Z[row + (size_t)(i0 - u) * ld];
 #pragma unroll
-            for (int t = 0; t < CH; ++t) {
-              if (t < cnt) {
-                const int i = i0 - t;
-                const R c = sh.cs[i], s = sh.sn[i];
-                Z[row + (size_t)(i + 1) * ld] = s * zc[t] + c * carry;
-                carry = c * zc[t] - s * carry;
-              }
-            }
+        for (int u = 0; u < CH; ++u) {
+          if (u < cnt) {
+            const int i = i0 - u;
+            const R c = cs[i], s = sn[i];
+            Z[row + (size_t)(i + 1) * ld] = s * zc[u] + c * carry;
+            carry = c * zc[u] - s * carry;
           }
-          Z[row + (size_t)l * ld] = carry;
-        }
-        const bool done = sh.iscal[3];
-        __syncthreads();
-        if (done) break;
-        if (++iter > 30 * n) {
-          if (tid == 0) sh.iscal[2] = 1;
-          break;
         }
       }
-    }
-    if (tid == 0) {
-      sh.dd[l] += sh.scal[1];
-      sh.ee[l] = 0;
+      Z[row + (size_t)l * ld] = carry;
     }
     __syncthreads();
   }
-  if (sh.iscal[2]) {
-    for (int jj = tid; jj < nmax; jj += NT) dout[jj] = 0;
+  if (sh.ql.fail) {
+    for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
     if (tid == 0) p.info[sid] = -1;  // QL failed to converge
     __syncthreads();
     return;
   }
+  eig_mark(p, sid, 5);
 
   // ---- ascending order: rank sort --------------------------------------------------
-  for (int i = tid; i < n; i += NT) {
+  for (int i = tid; i < n; i += NT_) {
     const R di = sh.dd[i];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -503,10 +612,10 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     sh.perm[rank] = i;
   }
   __syncthreads();
-  for (int j = tid; j < nmax; j += NT) dout[j] = j < n ? sh.dd[sh.perm[j]] : R(0);
+  for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.dd[sh.perm[j]] : R(0);
 
   // ---- back-transform X = Q Z[:, perm] ------------------------------------------------
-  for (int j = wave; j < n; j += NW) {
+  for (int j = wave; j < n; j += NW_) {
     const int pj = sh.perm[j];
     for (int i = lane; i < n; i += 64) X[i + (size_t)j * ldx] = mk<R>(Z[i + (size_t)pj * ld], 0);
   }
@@ -514,33 +623,63 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   for (int c = n - 2; c >= 0; --c) {
     const cplx<R> tau = A[c + (size_t)c * ld];
     if (tau.re == R(0) && tau.im == R(0)) continue;  // uniform: every thread reads the same value
-    for (int i = c + 1 + tid; i < n; i += NT)
+    for (int i = c + 1 + tid; i < n; i += NT_)
       sh.v[i] = (i == c + 1) ? mk<R>(1, 0) : A[i + (size_t)c * ld];
     __syncthreads();
-    for (int j = wave; j < n; j += NW) {
-      cplx<R>* col = X + (size_t)j * ldx;
-      R pr = 0, pi = 0;
-      for (int i = c + 1 + lane; i < n; i += 64) {
-        cplx<R> t = cmul_conj_a(sh.v[i], col[i]);
-        pr += t.re;
-        pi += t.im;
+    // X[c+1:, j] -= tau v (v^H X[c+1:, j]); UC columns per wave loaded together
+    for (int jb = wave; jb < n; jb += NW_ * UC) {
+      cplx<R> x[UC][RM];
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int j = jb + u * NW_;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          const int i = c + 1 + lane + 64 * r;
+          if (j < n && i < n) x[u][r] = X[i + (size_t)j * ldx];
+        }
       }
-      pr = wave_sum(pr);
-      pi = wave_sum(pi);
-      const cplx<R> t = tau * mk<R>(pr, pi);
-      for (int i = c + 1 + lane; i < n; i += 64) col[i] = col[i] - sh.v[i] * t;
+      cplx<R> t[UC];
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int j = jb + u * NW_;
+        R pr = 0, pi = 0;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          const int i = c + 1 + lane + 64 * r;
+          if (j < n && i < n) {
+            const cplx<R> q = cmul_conj_a(sh.v[i], x[u][r]);
+            pr += q.re;
+            pi += q.im;
+          }
+        }
+        pr = wave_sum(pr);
+        pi = wave_sum(pi);
+        t[u] = tau * mk<R>(pr, pi);
+      }
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int j = jb + u * NW_;
+        if (j >= n) continue;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          const int i = c + 1 + lane + 64 * r;
+          if (i < n) X[i + (size_t)j * ldx] = x[u][r] - sh.v[i] * t[u];
+        }
+      }
     }
     __syncthreads();
   }
 
+  eig_mark(p, sid, 6);
   // ---- un-whiten: eigenvectors of the pencil = L^-H X ---------------------------------
   if (!p.standard) bwd_solve_h<R, NMAX>(n, L, ld, X, ldx, sh);
+  eig_mark(p, sid, 7);
   if (tid == 0) p.info[sid] = 0;
   __syncthreads();
 }
 
 template <typename R, typename TIn, int NMAX>
-__global__ __launch_bounds__(NT) void eig_kernel(EigParams<R> p) {
+__global__ __launch_bounds__(EigCfg<NMAX>::NT) void eig_kernel(EigParams<R> p) {
   __shared__ EigShared<R, NMAX> sh;
   __shared__ int job_sh;
   for (;;) {
@@ -666,9 +805,9 @@ void launch_eig(const EigParams<R>& p, int n_slots, bool in_double, hipStream_t 
 #define BIPP_EIG_CASE(NM)                                                               \
   if (b == NM) {                                                                        \
     if (in_double)                                                                      \
-      eig_kernel<R, double, NM><<<n_slots, NT, 0, stream>>>(p);                         \
+      eig_kernel<R, double, NM><<<n_slots, EigCfg<NM>::NT, 0, stream>>>(p);             \
     else                                                                                \
-      eig_kernel<R, float, NM><<<n_slots, NT, 0, stream>>>(p);                          \
+      eig_kernel<R, float, NM><<<n_slots, EigCfg<NM>::NT, 0, stream>>>(p);              \
   }
   BIPP_EIG_CASE(64)
   BIPP_EIG_CASE(128)
@@ -711,17 +850,20 @@ int eig_blocks_per_cu(int nmax, bool in_double) {
   const int b = eig_nmax_bucket<R>(nmax);
   int nb = 1;
   const void* fn = nullptr;
+  int nt = NT;
 #define BIPP_EIG_FN(NM)                                                                      \
-  if (b == NM)                                                                               \
+  if (b == NM) {                                                                             \
     fn = in_double ? reinterpret_cast<const void*>(&eig_kernel<R, double, NM>)               \
-                   : reinterpret_cast<const void*>(&eig_kernel<R, float, NM>);
+                   : reinterpret_cast<const void*>(&eig_kernel<R, float, NM>);               \
+    nt = EigCfg<NM>::NT;                                                                     \
+  }
   BIPP_EIG_FN(64)
   BIPP_EIG_FN(128)
   BIPP_EIG_FN(256)
   BIPP_EIG_FN(512)
 #undef BIPP_EIG_FN
   if (!fn) return 1;
-  BIPP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, 0));
+  BIPP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nt, 0));
   return nb > 0 ? nb : 1;
 }
 template int eig_blocks_per_cu<float>(int, bool);

@@ -50,6 +50,8 @@ struct EigParams {
   long long ws_stride = 0;
   int ws_ld = 0;
   int* info = nullptr;
+  // optional phase timestamps (wall clock ticks), [n_jobs][8] by sample index
+  unsigned long long* prof = nullptr;
 };
 
 template <typename R>

@@ -30,7 +30,9 @@ namespace bippamd {
 
 namespace {
 
-constexpr int SP_NT = 256;  // spread workgroup
+constexpr int SP_NT = 256;   // spread workgroup: 2 cells of an 8^3 block per thread
+constexpr int SP_BIN = 8;    // spread block edge
+constexpr int SP_LMAX = 8;   // levels per spread launch (sums kept in registers)
 
 template <typename R>
 __device__ __forceinline__ R es_kernel(R z, R beta) {
@@ -65,7 +67,7 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   if ((long long)a2 * (a2 + 1) / 2 > t) --a2;
   const int b = static_cast<int>(t - (long long)a2 * (a2 + 1) / 2);
   const long long pid = (long long)s * npair + t;
-  if (a2 >= n) {
+  if (a2 >= n || (p.merge_auto && a2 == b && a2 > 0)) {
     p.valid[pid] = 0;
     return;
   }
@@ -104,7 +106,8 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
   pt.ph[0] = static_cast<float>(pc);
   pt.ph[1] = static_cast<float>(ps);
-  pt.src = static_cast<int>(a2 + p.ld_vv * b);  // VV(a, b) inside the sample's matrix
+  // VV(a, b) inside the sample's matrix; the merged a == b point reads the trace slot (0, 1)
+  pt.src = (p.merge_auto && a2 == b) ? static_cast<int>(p.ld_vv) : static_cast<int>(a2 + p.ld_vv * b);
   pt.mult = a2 == b ? 1.f : 2.f;                 // (a,b) and (b,a) are conjugate: 2 Re(.)
   pt.sample = s;
   pt.pad = 0;
@@ -148,134 +151,142 @@ __global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, c
 }
 
 // --------------------------------------------------------------------------------------
-// Spreading with output ownership.  The workgroup of block (bx, by, bz) owns the cells
-// l + nf1/2 in [b B, b B + B) per axis.  It walks the bins that can reach the block (a point
-// of bin c covers cells [l0, l0 + w), l0 in bin c), clips each point's support to the block,
-// accumulates kernel weight x value for every level in an LDS tile, and finally writes the
-// tile times the deconvolution factors with plain stores.  Every cell of [-nf1/2, nf1/2)^3
-// is written by exactly one workgroup; no global atomics, no zero pass over the grid.
+// Spreading with output ownership.  The workgroup of block (bx, by, bz) owns the 8^3 cells
+// l + nf1/2 in [8 b, 8 b + 8) per axis; thread t owns the two cells
+// (2 (t mod 4) + {0, 1}, (t / 4) mod 8, t / 32) and keeps their sums for every level in
+// registers.  The workgroup walks the bins that can reach the block (a point of bin c covers
+// cells [l0, l0 + w), l0 in bin c), a batch of PB points at a time: each point that meets
+// the block is compacted into LDS with its kernel weights per axis and its values, then
+// every thread adds the points that cover its cells.  The block is written once, times the
+// deconvolution factors, with plain stores: no atomics, a fixed summation order, and every
+// cell of [-nf1/2, nf1/2)^3 written by exactly one workgroup (no zero pass over the grid).
 // --------------------------------------------------------------------------------------
-constexpr int SP_PB = 64;  // points per batch (one wavefront scans the item counts)
-
 template <typename R, typename TV>
-__global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p) {
+__global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, int PB) {
   extern __shared__ __align__(16) unsigned char sp_sh[];
+  constexpr int B = SP_BIN;
   const int w = p.w;
-  const int BX = p.bin_size[0], BY = p.bin_size[1], BZ = p.bin_size[2];
-  const int tcells = BX * BY * BZ;
   const int L = p.n_levels;
-  R* tile = reinterpret_cast<R*>(sp_sh);                          // [L][tcells][2]
-  R* wts = tile + (size_t)2 * L * tcells;                         // [PB][3][w]
-  R* val = wts + (size_t)SP_PB * 3 * w;                           // [PB][L][2]
-  int* info = reinterpret_cast<int*>(val + (size_t)SP_PB * L * 2);  // [PB][8]
-  int* pref = info + SP_PB * 8;                                   // [PB + 1]
+  int* porg = reinterpret_cast<int*>(sp_sh);                     // [PB][4] l0 - block origin
+  R* pw = reinterpret_cast<R*>(porg + (size_t)PB * 4);           // [PB][3][w] kernel weights
+  R* pv = pw + (size_t)PB * 3 * w;                               // [PB][L][2] values
+  __shared__ int wave_cnt[SP_NT / 64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bin = blockIdx.x;
   const int bx = bin % p.nbins[0];
   const int by = (bin / p.nbins[0]) % p.nbins[1];
   const int bz = bin / (p.nbins[0] * p.nbins[1]);
-  const int org[3] = {bx * BX, by * BY, bz * BZ};
-  const int bsz[3] = {BX, BY, BZ};
-  for (int i = threadIdx.x; i < 2 * L * tcells; i += SP_NT) tile[i] = R(0);
+  const int org[3] = {bx * B, by * B, bz * B};
+  const int cx0 = 2 * (tid & 3), cy = (tid >> 2) & 7, cz = tid >> 5;
+  R acc[2][SP_LMAX][2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int lv = 0; lv < SP_LMAX; ++lv) acc[c][lv][0] = acc[c][lv][1] = R(0);
+  const unsigned long long t_start = p.prof ? wall_clock64() : 0;
+  unsigned long long n_visit = 0, n_hit = 0;
   const R two_over_w = R(2) / R(w);
   const R beta = R(p.beta);
-  for (int cz = max(0, bz - p.nback[2]); cz <= bz; ++cz)
-    for (int cy = max(0, by - p.nback[1]); cy <= by; ++cy)
-      for (int cx = max(0, bx - p.nback[0]); cx <= bx; ++cx) {
-        const int nbid = (cz * p.nbins[1] + cy) * p.nbins[0] + cx;
+  for (int cz_ = max(0, bz - p.nback[2]); cz_ <= bz; ++cz_)
+    for (int cy_ = max(0, by - p.nback[1]); cy_ <= by; ++cy_)
+      for (int cx_ = max(0, bx - p.nback[0]); cx_ <= bx; ++cx_) {
+        const int nbid = (cz_ * p.nbins[1] + cy_) * p.nbins[0] + cx_;
         const unsigned beg = p.bin_start[nbid];
         const unsigned cnt = p.bin_count[nbid];
-        for (unsigned b0 = 0; b0 < cnt; b0 += SP_PB) {
-          const int nb = min<unsigned>(SP_PB, cnt - b0);
+        for (unsigned b0 = 0; b0 < cnt; b0 += PB) {
+          const int nb = min<unsigned>(PB, cnt - b0);
+          // ---- compact the points of the batch that meet the block --------------------
+          bool hit = false;
+          N3Point pt;
+          int rel[3] = {0, 0, 0};
+          if (tid < nb) {
+            pt = p.pts[beg + b0 + tid];
+            hit = true;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              rel[d] = pt.l0[d] - org[d];
+              hit = hit && rel[d] < B && rel[d] + w > 0;
+            }
+          }
+          const unsigned long long mask = __ballot(hit);
           __syncthreads();  // previous batch fully consumed
-          // support of each point clipped to the block
-          if (threadIdx.x < SP_PB) {
-            int items = 0;
-            if ((int)threadIdx.x < nb) {
-              const N3Point& pt = p.pts[beg + b0 + threadIdx.x];
-              items = 1;
-              for (int d = 0; d < 3; ++d) {
-                const int lo = max(pt.l0[d], org[d]);
-                const int hi = min(pt.l0[d] + w, org[d] + bsz[d]);
-                const int c = max(0, hi - lo);
-                info[threadIdx.x * 8 + d] = lo - pt.l0[d];      // first kernel offset
-                info[threadIdx.x * 8 + 3 + d] = c;              // offsets inside the block
-                items *= c;
-              }
-              info[threadIdx.x * 8 + 6] = pt.l0[0] - org[0];
-              info[threadIdx.x * 8 + 7] = (pt.l0[1] - org[1]) * BX + (pt.l0[2] - org[2]) * BX * BY;
-            }
-            // inclusive scan over the wavefront
-            int v = items;
-            for (int o = 1; o < SP_PB; o <<= 1) {
-              const int u = __shfl_up(v, o, SP_PB);
-              if ((int)threadIdx.x >= o) v += u;
-            }
-            pref[threadIdx.x + 1] = v;
-            if (threadIdx.x == 0) pref[0] = 0;
-          }
+          if (lane == 0) wave_cnt[wave] = __popcll(mask);
           __syncthreads();
-          const int total = pref[nb];
-          if (total == 0) continue;
-          // kernel weights of the clipped offsets, values of every level
-          for (int t = threadIdx.x; t < nb * 3 * w; t += SP_NT) {
-            const int pi = t / (3 * w), rem = t % (3 * w), d = rem / w, j = rem % w;
-            if (j >= info[pi * 8 + 3 + d]) continue;
-            const N3Point& pt = p.pts[beg + b0 + pi];
-            const int o = info[pi * 8 + d] + j;
-            wts[(pi * 3 + d) * w + j] = es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
+          int slot = __popcll(mask & ((1ull << lane) - 1ull));
+          int total = 0;
+          for (int q = 0; q < SP_NT / 64; ++q) {
+            if (q < wave) slot += wave_cnt[q];
+            total += wave_cnt[q];
           }
-          for (int t = threadIdx.x; t < nb * L; t += SP_NT) {
-            const int pi = t / L, lv = t % L;
-            if (pref[pi + 1] == pref[pi]) continue;
-            const N3Point& pt = p.pts[beg + b0 + pi];
-            const TV* vv =
-                p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0 + lv) * p.vv_stride;
-            const TV c = vv[pt.src];
+          if (hit) {
+            porg[slot * 4 + 0] = rel[0];
+            porg[slot * 4 + 1] = rel[1];
+            porg[slot * 4 + 2] = rel[2];
+            for (int d = 0; d < 3; ++d)
+              for (int o = 0; o < w; ++o)
+                pw[((size_t)slot * 3 + d) * w + o] =
+                    es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
+            const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
             const R m = R(pt.mult);
             const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
-            val[(pi * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
-            val[(pi * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+            for (int lv = 0; lv < L; ++lv) {
+              const TV c = vv[(size_t)lv * p.vv_stride + pt.src];
+              pv[((size_t)slot * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
+              pv[((size_t)slot * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+            }
           }
           __syncthreads();
-          // items (point, clipped offset), x offsets fastest: consecutive lanes take
-          // consecutive cells
-          for (int it = threadIdx.x; it < total; it += SP_NT) {
-            int lo = 0, hi = nb;  // pref[lo] <= it < pref[hi]
-            while (hi - lo > 1) {
-              const int mid = (lo + hi) >> 1;
-              if (pref[mid] <= it) lo = mid;
-              else hi = mid;
-            }
-            const int pi = lo;
-            const int r = it - pref[pi];
-            const int c0 = info[pi * 8 + 3], c1 = info[pi * 8 + 4];
-            const int jx = r % c0, rq = r / c0, jy = rq % c1, jz = rq / c1;
-            const R wt = wts[(pi * 3 + 0) * w + jx] * wts[(pi * 3 + 1) * w + jy] *
-                         wts[(pi * 3 + 2) * w + jz];
-            // tile cell of kernel offset (info0 + jx, info1 + jy, info2 + jz)
-            const int cell = info[pi * 8 + 6] + info[pi * 8 + 0] + jx +
-                             (info[pi * 8 + 1] + jy) * BX + (info[pi * 8 + 2] + jz) * BX * BY +
-                             info[pi * 8 + 7];
-            for (int lv = 0; lv < L; ++lv) {
-              atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 0], wt * val[(pi * L + lv) * 2 + 0]);
-              atomicAdd(&tile[((size_t)lv * tcells + cell) * 2 + 1], wt * val[(pi * L + lv) * 2 + 1]);
+          n_visit += nb;
+          n_hit += total;
+          // ---- every thread adds the points covering its two cells ----------------------
+          for (int q = 0; q < total; ++q) {
+            const int ly = cy - porg[q * 4 + 1], lz = cz - porg[q * 4 + 2];
+            if ((unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w) {
+              const R wyz = pw[((size_t)q * 3 + 1) * w + ly] * pw[((size_t)q * 3 + 2) * w + lz];
+              const int lx = cx0 - porg[q * 4 + 0];
+              const R w0 = (unsigned)lx < (unsigned)w ? pw[(size_t)q * 3 * w + lx] * wyz : R(0);
+              const R w1 =
+                  (unsigned)(lx + 1) < (unsigned)w ? pw[(size_t)q * 3 * w + lx + 1] * wyz : R(0);
+              const R* v = pv + (size_t)q * L * 2;
+#pragma unroll
+              for (int lv = 0; lv < SP_LMAX; ++lv) {
+                if (lv < L) {
+                  const R vr = v[lv * 2], vi = v[lv * 2 + 1];
+                  acc[0][lv][0] += w0 * vr;
+                  acc[0][lv][1] += w0 * vi;
+                  acc[1][lv][0] += w1 * vr;
+                  acc[1][lv][1] += w1 * vi;
+                }
+              }
             }
           }
         }
       }
-  __syncthreads();
-  // deconvolve and store the owned cells inside [-nf1/2, nf1/2)^3 (grid index l mod nf2)
-  for (int i = threadIdx.x; i < L * tcells; i += SP_NT) {
-    const int lv = i / tcells, cell = i % tcells;
-    const int ix = org[0] + cell % BX, iy = org[1] + (cell / BX) % BY, iz = org[2] + cell / (BX * BY);
-    if (ix >= p.nf1[0] || iy >= p.nf1[1] || iz >= p.nf1[2]) continue;
-    const R f = p.fx[ix] * p.fy[iy] * p.fz[iz];
-    int gx = ix - p.nf1[0] / 2, gy = iy - p.nf1[1] / 2, gz = iz - p.nf1[2] / 2;
+  if (p.prof && tid == 0) {
+    unsigned long long* pr = p.prof + (size_t)bin * 4;
+    pr[0] = t_start;
+    pr[1] = wall_clock64();
+    pr[2] = n_visit;
+    pr[3] = n_hit;
+  }
+  // ---- deconvolve and store the owned cells inside [-nf1/2, nf1/2)^3 ------------------
+  const int iy = org[1] + cy, iz = org[2] + cz;
+  if (iy >= p.nf1[1] || iz >= p.nf1[2]) return;
+  int gy = iy - p.nf1[1] / 2, gz = iz - p.nf1[2] / 2;
+  gy += gy < 0 ? p.nf2[1] : 0;
+  gz += gz < 0 ? p.nf2[2] : 0;
+  const R fyz = p.fy[iy] * p.fz[iz];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int ix = org[0] + cx0 + c;
+    if (ix >= p.nf1[0]) continue;
+    const R f = p.fx[ix] * fyz;
+    int gx = ix - p.nf1[0] / 2;
     gx += gx < 0 ? p.nf2[0] : 0;
-    gy += gy < 0 ? p.nf2[1] : 0;
-    gz += gz < 0 ? p.nf2[2] : 0;
-    p.grid[(size_t)(p.lv0 + lv) * p.grid_stride + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx] =
-        mk<R>(tile[(size_t)i * 2] * f, tile[(size_t)i * 2 + 1] * f);
+    cplx<R>* g = p.grid + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx;
+#pragma unroll
+    for (int lv = 0; lv < SP_LMAX; ++lv)
+      if (lv < L) g[(size_t)(p.lv0 + lv) * p.grid_stride] = mk<R>(acc[c][lv][0] * f, acc[c][lv][1] * f);
   }
 }
 
@@ -284,13 +295,13 @@ __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p) {
 // Returns the buffer holding the result.
 // --------------------------------------------------------------------------------------
 // Small DFTs with sign +, constant roots.
-template <typename R>
-__device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
-  if (rad == 2) {
+template <typename R, int RAD>
+__device__ __forceinline__ void dft_small(cplx<R>* v) {
+  if constexpr (RAD == 2) {
     const cplx<R> a = v[0], b = v[1];
     v[0] = a + b;
     v[1] = a - b;
-  } else if (rad == 4) {
+  } else if constexpr (RAD == 4) {
     const cplx<R> a = v[0], b = v[1], c = v[2], d = v[3];
     const cplx<R> s0 = a + c, s1 = a - c, s2 = b + d, s3 = b - d;
     const cplx<R> is3 = mk<R>(-s3.im, s3.re);  // +i * s3
@@ -298,7 +309,7 @@ __device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
     v[1] = s1 + is3;
     v[2] = s0 - s2;
     v[3] = s1 - is3;
-  } else if (rad == 3) {
+  } else if constexpr (RAD == 3) {
     // w = exp(+2 pi i / 3): y1,2 = a - (b + c)/2 +- i sqrt(3)/2 (b - c)
     const R h = R(0.86602540378443864676);
     const cplx<R> a = v[0], t1 = v[1] + v[2], t2 = v[1] - v[2];
@@ -308,6 +319,7 @@ __device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
     v[1] = m + q;
     v[2] = m - q;
   } else {
+    static_assert(RAD == 5, "radix 2, 3, 4 or 5");
     // radix 5: roots exp(+2 pi i k / 5)
     const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
     const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
@@ -329,28 +341,62 @@ __device__ __forceinline__ void dft_small(cplx<R>* v, int rad) {
   }
 }
 
-// Stockham stages in LDS; `tw` holds exp(+2 pi i t / n) for t < n.
+// exp(+2 pi i t / n) from two LDS tables: t1[t mod 64] * t2[t / 64]
+template <typename R>
+struct Twid {
+  const cplx<R>* t1;
+  const cplx<R>* t2;
+  __device__ __forceinline__ cplx<R> operator()(int t) const { return t1[t & 63] * t2[t >> 6]; }
+};
+
+// table layout in global memory: [64] fine entries, then [ceil(n / 64)] coarse entries
+__host__ __device__ __forceinline__ int twid_len(int n) { return 64 + (n + 63) / 64; }
+
+template <typename R>
+__device__ __forceinline__ Twid<R> load_twid(const cplx<R>* __restrict__ g, int n, cplx<R>* lds,
+                                             int tid, int nt) {
+  const int len = twid_len(n);
+  for (int i = tid; i < len; i += nt) lds[i] = g[i];
+  return Twid<R>{lds, lds + 64};
+}
+
+// One Stockham stage of radix RAD over nl lines (stride ld) of length n.
+template <typename R, int RAD>
+__device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n, int nl, int ld,
+                                          int ns, const Twid<R>& tw, int tid, int nt) {
+  const int nb = n / RAD;
+  const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
+  for (int t = tid; t < nl * nb; t += nt) {
+    const int line = t / nb, j = t - line * nb;
+    const cplx<R>* src = in + (size_t)line * ld;
+    cplx<R>* dst = out + (size_t)line * ld;
+    const int jq = j / ns;
+    const int k = j - jq * ns;
+    cplx<R> v[RAD];
+    v[0] = src[j];
+#pragma unroll
+    for (int r = 1; r < RAD; ++r) v[r] = src[j + r * nb] * tw(k * r * tstride);
+    dft_small<R, RAD>(v);
+    const int idx = jq * ns * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) dst[idx + r * ns] = v[r];
+  }
+}
+
+// Stockham stages in LDS; returns the buffer holding the result.
 template <typename R>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
-                            const cplx<R>* __restrict__ tw, int tid, int nt) {
+                            const Twid<R>& tw, int tid, int nt) {
   int ns = 1;
   cplx<R>* in = a;
   cplx<R>* out = b;
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
-    const int nb = n / R_;
-    const int tstride = n / (ns * R_);  // twiddle k r / (ns R) of the full length
-    for (int t = tid; t < nl * nb; t += nt) {
-      const int line = t / nb, j = t % nb;
-      const cplx<R>* src = in + (size_t)line * ld;
-      cplx<R>* dst = out + (size_t)line * ld;
-      const int k = j % ns;
-      cplx<R> v[5];
-      v[0] = src[j];
-      for (int r = 1; r < R_; ++r) v[r] = src[j + r * nb] * tw[k * r * tstride];
-      dft_small<R>(v, R_);
-      const int idx = (j / ns) * ns * R_ + k;
-      for (int r = 0; r < R_; ++r) dst[idx + r * ns] = v[r];
+    switch (R_) {
+      case 2: fft_stage<R, 2>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      case 3: fft_stage<R, 3>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      case 4: fft_stage<R, 4>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      default: fft_stage<R, 5>(in, out, n, nl, ld, ns, tw, tid, nt); break;
     }
     __syncthreads();
     ns *= R_;
@@ -362,7 +408,7 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
 }
 
 // position of index a inside window [start, start + count) of an axis of length n
-__device__ __forceinline__ bool in_window(int a, const N3Window& wd, int n) {
+__device__ __forceinline__ bool in_window(int a, N3Window wd, int n) {
   int d = a - wd.start;
   d += d < 0 ? n : 0;
   return d < wd.count;
@@ -376,6 +422,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int n
   const int n = p.n;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
   cplx<R>* b = a + (size_t)nl * n;
+  const Twid<R> tw = load_twid<R>(p.tw, n, b + (size_t)nl * n, threadIdx.x, blockDim.x);
   const long long n_lines = (long long)p.n_levels * p.zw.count * p.yw.count;
   const long long n_groups = (n_lines + nl - 1) / nl;
   for (long long grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
@@ -398,7 +445,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int n
       a[i] = in_window(x, p.in, n) ? p.grid[lbase[li] + x] : mk<R>(0, 0);
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, p.tw, threadIdx.x, blockDim.x);
+    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, tw, threadIdx.x, blockDim.x);
     for (int i = threadIdx.x; i < nlg * n; i += blockDim.x) {
       const int li = i / n, k = i - li * n;
       if (in_window(k, p.out, n)) p.grid[lbase[li] + k] = res[i];
@@ -417,6 +464,12 @@ __global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
   const int ld = p.m | 1;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
   cplx<R>* b = a + TW * ld;
+  cplx<R>* tws = b + TW * ld;
+  const Twid<R> tw = load_twid<R>(p.tw, p.m, tws, threadIdx.x, blockDim.x);
+  // second table (two-pass twiddles) right after the first; unused when p.twiddle == 0
+  cplx<R>* tws2 = tws + twid_len(p.m);
+  if (p.twiddle) load_twid<R>(p.tw2, static_cast<int>(p.ntot), tws2, threadIdx.x, blockDim.x);
+  const Twid<R> tw2{tws2, tws2 + 64};
   const int n_xt = (p.xw.count + TW - 1) / TW;
   const long long n_items = (long long)n_xt * p.n_sub * p.other.count * p.n_levels;
   for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
@@ -444,7 +497,7 @@ __global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
       a[xi * ld + i] = v;
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, p.m, TW, ld, p.rad, p.tw, threadIdx.x, blockDim.x);
+    cplx<R>* res = lds_fft<R>(a, b, p.m, TW, ld, p.rad, tw, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < TW * p.m; t += blockDim.x) {
       const int xi = t % TW, k = t / TW;
       const int j = xt * TW + xi;
@@ -453,7 +506,7 @@ __global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
       int x = p.xw.start + j;
       x -= x >= p.nx ? p.nx : 0;
       cplx<R> v = res[xi * ld + k];
-      if (p.twiddle) v = v * p.tw2[((long long)sub * k) % p.ntot];
+      if (p.twiddle) v = v * tw2(static_cast<int>(((long long)sub * k) % p.ntot));
       const long long ai = (long long)sub * p.sub_stride + (long long)k * p.estride;
       base[ai * p.axis_stride + x] = v;
     }
@@ -585,6 +638,27 @@ __global__ __launch_bounds__(256) void vv_levels_kernel(N3VVParams<R> p) {
   }
 }
 
+// Trace of VV_l for every (sample, level) into the unused upper-triangle slot (0, 1).
+template <typename TO>
+__global__ __launch_bounds__(256) void vv_trace_kernel(int n_levels, const int* n_rows,
+                                                       int n_rows_const, cplx<TO>* vv,
+                                                       long long vv_stride, long long ld_vv) {
+  __shared__ double red[4];
+  const int sl = blockIdx.x;
+  const int s = sl / n_levels;
+  const int n = n_rows ? n_rows[s] : n_rows_const;
+  cplx<TO>* m = vv + (size_t)sl * vv_stride;
+  double re = 0, im = 0;
+  for (int a = threadIdx.x; a < n; a += 256) {
+    const cplx<TO> c = m[a + ld_vv * a];
+    re += static_cast<double>(c.re);
+    im += static_cast<double>(c.im);
+  }
+  re = block_sum<256>(re, red);
+  im = block_sum<256>(im, red);
+  if (threadIdx.x == 0) m[ld_vv] = mk<TO>(static_cast<TO>(re), static_cast<TO>(im));
+}
+
 }  // namespace
 
 // --------------------------------------------------------------------------------------
@@ -645,10 +719,8 @@ void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* b
 }
 
 template <typename R>
-size_t n3_spread_shmem(const N3SpreadParams<R>& p) {
-  const size_t tcells = (size_t)p.bin_size[0] * p.bin_size[1] * p.bin_size[2];
-  return sizeof(R) * (2 * p.n_levels * tcells + SP_PB * 3 * p.w + SP_PB * p.n_levels * 2) +
-         sizeof(int) * (SP_PB * 8 + SP_PB + 1);
+size_t n3_spread_shmem(int pb, int w, int n_levels) {
+  return sizeof(int) * 4 * (size_t)pb + sizeof(R) * (size_t)pb * (3 * w + 2 * n_levels);
 }
 
 template <typename R, typename TV>
@@ -657,29 +729,27 @@ void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
   if (n_blocks <= 0) return;
   if (n_blocks > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT spread blocks");
   for (int d = 0; d < 3; ++d)
-    if ((long long)p_all.nbins[d] * p_all.bin_size[d] < p_all.nf1[d] || p_all.nback[d] < 0)
+    if (p_all.bin_size[d] != SP_BIN || (long long)p_all.nbins[d] * SP_BIN < p_all.nf1[d] ||
+        p_all.nback[d] < 0)
       fail(BIPP_INTERNAL_ERROR, "NUFFT spread blocks do not cover the grid");
   static bool attr_set = false;
   if (!attr_set) {
     BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&n3_spread_kernel<R, TV>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attr_set = true;
   }
-  // as many levels per launch as the LDS tile allows
   const int L = p_all.n_levels_total;
-  int group = L;
-  for (; group > 1; --group) {
-    N3SpreadParams<R> t = p_all;
-    t.n_levels = group;
-    if (n3_spread_shmem<R>(t) <= 64 * 1024) break;
-  }
+  const int group = std::min(L, SP_LMAX);
+  // points per batch: as many as the LDS budget allows (one per thread at most)
+  int pb = SP_NT;
+  while (pb > 64 && n3_spread_shmem<R>(pb, p_all.w, group) > 64 * 1024) pb /= 2;
   for (int lv0 = 0; lv0 < L; lv0 += group) {
     N3SpreadParams<R> p = p_all;
     p.lv0 = lv0;
     p.n_levels = std::min(group, L - lv0);
-    const size_t sh = n3_spread_shmem<R>(p);
-    if (sh > 160 * 1024) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT spread tile exceeds LDS");
-    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), SP_NT, sh, st>>>(p);
+    const size_t sh = n3_spread_shmem<R>(pb, p.w, p.n_levels);
+    if (sh > 150 * 1024) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT spread batch exceeds LDS");
+    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), SP_NT, sh, st>>>(p, pb);
     check_launch("n3_spread_kernel");
   }
 }
@@ -701,7 +771,7 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
     fail(BIPP_INTERNAL_ERROR, "FFT line windows out of range");
   // several short lines per workgroup so that every stage keeps most lanes busy
   const int nl = std::max(1, std::min(16, 2048 / p.n));
-  const size_t sh = 2 * sizeof(cplx<R>) * (size_t)p.n * nl;
+  const size_t sh = sizeof(cplx<R>) * (2 * (size_t)p.n * nl + twid_len(p.n));
   // 1 KiB of the 160 KiB stays for the kernel's static line bases
   if (sh > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
   static bool attr_set = false;
@@ -726,7 +796,8 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
   if (!window_ok(p.xw, p.nx) || !window_ok(p.other, p.no) || !window_ok(p.in, p.nax) ||
       !window_ok(p.out, p.nax) || (long long)p.m * p.n_sub != p.nax)
     fail(BIPP_INTERNAL_ERROR, "FFT pass windows out of range");
-  const size_t sh = 2 * 16 * sizeof(cplx<R>) * (size_t)(p.m | 1);
+  const size_t sh = sizeof(cplx<R>) * (2 * 16 * (size_t)(p.m | 1) + twid_len(p.m) +
+                                       (p.twiddle ? twid_len(static_cast<int>(p.ntot)) : 0));
   if (sh > 160 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");
   static bool attr_set = false;
   if (!attr_set) {
@@ -750,6 +821,20 @@ void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st) {
 }
 template void launch_n3_interp<float, float>(const N3InterpParams<float>&, hipStream_t);
 template void launch_n3_interp<double, double>(const N3InterpParams<double>&, hipStream_t);
+
+template <typename TO>
+void launch_vv_trace(int n_samples, int n_levels, const int* n_rows, int n_rows_const, void* vv,
+                     long long vv_stride, long long ld_vv, hipStream_t st) {
+  if (n_samples <= 0 || n_levels <= 0) return;
+  if (ld_vv < 2) fail(BIPP_INTERNAL_ERROR, "trace slot needs at least two rows");
+  vv_trace_kernel<TO><<<n_samples * n_levels, 256, 0, st>>>(
+      n_levels, n_rows, n_rows_const, static_cast<cplx<TO>*>(vv), vv_stride, ld_vv);
+  check_launch("vv_trace_kernel");
+}
+template void launch_vv_trace<float>(int, int, const int*, int, void*, long long, long long,
+                                     hipStream_t);
+template void launch_vv_trace<double>(int, int, const int*, int, void*, long long, long long,
+                                      hipStream_t);
 
 template <typename R, typename TO>
 void launch_vv_levels(const N3VVParams<R>& p, int n_samples, int n_rows_max, hipStream_t st) {

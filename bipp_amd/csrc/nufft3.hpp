@@ -33,6 +33,9 @@ struct N3PointsParams {
   int off[3], bin_size[3], nbins[3];
   int phase_fp32;
   long long ld_vv;
+  // all pairs a == b sit at u = 0: emit one point per sample, valued by the trace of VV_l
+  // stored in the unused slot (0, 1) of each matrix
+  int merge_auto;
   // outputs, [nS * npair]
   unsigned char* valid;
   int* bin_of;
@@ -57,6 +60,8 @@ struct N3SpreadParams {
   const R *fx, *fy, *fz;         // 1/phi2^(l) for l = -nf1/2 .. nf1/2-1
   cplx<R>* grid;                 // [L][grid_stride]
   long long grid_stride;
+  // optional per-block profile [n_blocks][4]: start, end (wall clock), points, items
+  unsigned long long* prof;
 };
 
 // Window of an axis of length n: indices (start + i) mod n for i < count.
@@ -153,6 +158,9 @@ template <typename R>
 void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st);
 template <typename R, typename TI>
 void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st);
+template <typename TO>
+void launch_vv_trace(int n_samples, int n_levels, const int* n_rows, int n_rows_const, void* vv,
+                     long long vv_stride, long long ld_vv, hipStream_t st);
 template <typename R, typename TO>
 void launch_vv_levels(const N3VVParams<R>& p, int n_samples, int n_rows_max, hipStream_t st);
 
