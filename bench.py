@@ -30,7 +30,20 @@ from bipp_amd import hip, synthetic  # noqa: E402
 from bipp_amd.hip import DeviceArray  # noqa: E402
 
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+PEAK_FP64_TFLOPS = 78.6     # MI355X fp64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_traffic.json: FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration), or None when no such summary is committed."""
+    path = os.path.join(HERE, "profiles", f"{kernel}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
 IMAGER = {1: "steering", 2: "steering", 3: "steering", 4: "nufft", 5: "nufft"}
 
 
@@ -140,6 +153,9 @@ def main() -> None:
     ap.add_argument("--plain-samples", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="end with one device-to-device copy of BYTES (hardware-counter "
+                         "calibration for rocprofv3 --pmc runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the sample-parallel "
@@ -207,28 +223,65 @@ def main() -> None:
     ok = max(errs_m) <= 1e-4 and masks_equal
     value = n_s / t_m if ok else 0.0
 
+    # roofline of the dominant kernel of the timed step, plus the other stages beside it
+    _, n_red = syn_m.last_eigenvalues(n_s)
+    nr = n_red.astype(np.float64)
+    eig_ms = stages["eigensolve_ms"]
+    # eigensolver: every sample reads its complex64 pencil input (S) and antenna positions and
+    # writes complex128 eigenvectors and float64 eigenvalues; 40 N^3 real flops per sample
+    eig_bytes = float(np.sum(nr * nr * 8.0 + 24.0 * nr + nr * nr * 16.0 + 8.0 * nr))
+    eig_flops = float(np.sum(40.0 * nr ** 3))
+    eig_gbs = eig_bytes / (eig_ms * 1e-3) / 1e9
+    rl_eig = {"bound": "hbm", "achieved": eig_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+              "frac": eig_gbs / PEAK_HBM_GBS, "traffic": None,
+              "kernel": "eig_kernel (persistent batched generalized eigensolver)",
+              "kernel_ms": eig_ms, "algorithmic_bytes_per_launch": eig_bytes,
+              "flops_model_per_launch": eig_flops,
+              "achieved_tflops_fp64": eig_flops / (eig_ms * 1e-3) / 1e12,
+              "fp64_peak_tflops": PEAK_FP64_TFLOPS}
+    tr = measured_traffic("eig_kernel")
+    if tr:
+        rl_eig["traffic"] = tr["bytes_per_launch"]
+        rl_eig["traffic_source"] = tr["source"]
+    stage_rl = {}
     if imager == "steering":
         flops = float(np.sum(8.0 * n * k_eig * P))   # complex GEMM V^T E: 8 N K P per sample
         kern_ms = stages["image_ms"]
         achieved = flops / (kern_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-                    "kernel": "steer_kernel<BF16X3>", "kernel_ms": kern_ms,
-                    "algorithmic_flops_per_launch": flops,
-                    "mfma_flops_executed_per_launch": 3 * flops}
+        stage_rl["steer"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                             "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
+                             "kernel": "steer_kernel<BF16X3>", "kernel_ms": kern_ms,
+                             "algorithmic_flops_per_launch": flops,
+                             "mfma_flops_executed_per_launch": 3 * flops}
+        img_ms = kern_ms
     else:
+        # type-3 NUFFT chain (sort + spread + FFT + interpolation), algorithmic bytes: every
+        # point's coordinates (12 B) once and its value (8 B) per level, the spread grid
+        # (nf1 cells, complex64) written by the spread, read and written by the FFT and read
+        # by the interpolation per level, the pixels' lmn (12 B) and image RMW (4 B per level)
         nf = syn_m.nufft_grid()
-        cells = float(np.prod(nf))
+        nf1 = [x // 2 for x in nf]
         L = len(intervals)
-        n_pass = 1 + sum(1 if x <= 128 else 2 for x in nf[1:])
-        fft_bytes = cells * L * 8.0 * 2 * n_pass        # complex64 read + write per pass
-        kern_ms = stages["nufft_fft_ms"]
-        achieved = fft_bytes / (kern_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                    "kernel": "deconvolution + Stockham FFT passes (fft_x_kernel, fft_strided_kernel)",
-                    "kernel_ms": kern_ms, "grid": list(nf), "passes": n_pass,
-                    "algorithmic_bytes_per_launch": fft_bytes}
+        m_pts = float(np.sum(nr * (nr - 1) / 2 + 1))
+        g_cells = float(np.prod(nf1))
+        nufft_bytes = m_pts * (12.0 + 8.0 * L) + g_cells * 8.0 * 4 * L + P * (12.0 + 8.0 * L)
+        kern_ms = (stages["nufft_sort_ms"] + stages["nufft_spread_ms"] + stages["nufft_fft_ms"]
+                   + stages["nufft_interp_ms"])
+        achieved = nufft_bytes / (kern_ms * 1e-3) / 1e9
+        stage_rl["nufft"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                             "kernel": "n3_points/scatter, n3_spread_kernel, fft_lines/fft_pass "
+                                       "(pruned), n3_interp_kernel",
+                             "kernel_ms": kern_ms, "fine_grid": list(nf), "spread_grid": nf1,
+                             "points": m_pts, "algorithmic_bytes_per_launch": nufft_bytes}
+        img_ms = stages["image_ms"]
+    # the dominant kernel of the step is reported as "roofline"
+    if eig_ms >= img_ms or not stage_rl:
+        roofline = rl_eig
+    else:
+        key = next(iter(stage_rl))
+        roofline = dict(stage_rl.pop(key), traffic=None)
+        stage_rl["eigensolve"] = rl_eig
     cpu = None
     if not a.no_cpu:
         try:
@@ -259,8 +312,15 @@ def main() -> None:
         "speedup_vs_plain": (value / plain["samples_per_s"]) if (plain and value) else None,
         "stages_ms_per_step": stages,
         "roofline": roofline,
+        "roofline_stages": stage_rl,
         "cpu_baseline": cpu,
     }
+    if a.copy_calibration > 0:
+        src = hip.DeviceArray.zeros(a.copy_calibration, np.uint8)
+        dst = hip.DeviceArray(a.copy_calibration, np.uint8)
+        hip.copy_device(dst, src, a.copy_calibration)
+        hip.synchronize()
+        out["copy_calibration_bytes"] = a.copy_calibration
     print(json.dumps(out), flush=True)
 
 

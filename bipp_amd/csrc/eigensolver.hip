@@ -566,12 +566,14 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     if (!sh.sweep[cur][2]) break;
     const int l = sh.sweep[cur][0], m = sh.sweep[cur][1];
     if (tid == gen) sh.sweep[cur ^ 1][2] = ql_next_sweep<R, NMAX>(n, sh, cur ^ 1) ? 1 : 0;
+    // a sweep always satisfies 0 <= l < m < n; anything else is skipped, never indexed
+    const bool sweep_ok = l >= 0 && l < m && m < n;
     // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
     // fetched CH at a time so that their loads are in flight together
     constexpr int CH = 32;
     const R* cs = sh.cs[cur];
     const R* sn = sh.sn[cur];
-    for (int row = tid; row < n; row += NT_) {
+    for (int row = sweep_ok ? tid : n; row < n; row += NT_) {
       R carry = Z[row + (size_t)m * ld];
       for (int i0 = m - 1; i0 >= l; i0 -= CH) {
         const int cnt = min(CH, i0 - l + 1);
@@ -602,11 +604,14 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 5);
 
   // ---- ascending order: rank sort --------------------------------------------------
+  // (NaN ordered as +inf, ties by index: the ranks always form a permutation of [0, n))
   for (int i = tid; i < n; i += NT_) {
-    const R di = sh.dd[i];
+    const R di0 = sh.dd[i];
+    const R di = di0 == di0 ? di0 : std::numeric_limits<R>::infinity();
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const R dj = sh.dd[j];
+      const R dj0 = sh.dd[j];
+      const R dj = dj0 == dj0 ? dj0 : std::numeric_limits<R>::infinity();
       rank += (dj < di) || (dj == di && j < i);
     }
     sh.perm[rank] = i;

@@ -88,6 +88,11 @@ class DeviceArray:
             self.free()
 
 
+def copy_device(dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+    """Device-to-device copy of `nbytes` (the runtime's copy kernel)."""
+    _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def synchronize() -> None:
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
