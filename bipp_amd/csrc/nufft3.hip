@@ -33,6 +33,7 @@ namespace {
 constexpr int SP_NT = 256;   // spread workgroup: 2 cells of an 8^3 block per thread
 constexpr int SP_BIN = 8;    // spread block edge
 constexpr int SP_LMAX = 8;   // levels per spread launch (sums kept in registers)
+constexpr int SP_MAXR = 9;   // neighbour bin ranges per block: (nback + 1)^2 with nback <= 2
 
 template <typename R>
 __device__ __forceinline__ R es_kernel(R z, R beta) {
@@ -187,81 +188,92 @@ __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, i
   unsigned long long n_visit = 0, n_hit = 0;
   const R two_over_w = R(2) / R(w);
   const R beta = R(p.beta);
+  // The bins that can reach the block: for each (cy, cz) of the neighbourhood the bins
+  // cx = bx - nback .. bx are consecutive in the sorted point array, so the neighbourhood is
+  // at most (nback + 1)^2 contiguous ranges; batches run over their concatenation.
+  unsigned rbeg[SP_MAXR], rlen[SP_MAXR];
+  int nr = 0;
+  unsigned total_pts = 0;
   for (int cz_ = max(0, bz - p.nback[2]); cz_ <= bz; ++cz_)
-    for (int cy_ = max(0, by - p.nback[1]); cy_ <= by; ++cy_)
-      for (int cx_ = max(0, bx - p.nback[0]); cx_ <= bx; ++cx_) {
-        const int nbid = (cz_ * p.nbins[1] + cy_) * p.nbins[0] + cx_;
-        const unsigned beg = p.bin_start[nbid];
-        const unsigned cnt = p.bin_count[nbid];
-        for (unsigned b0 = 0; b0 < cnt; b0 += PB) {
-          const int nb = min<unsigned>(PB, cnt - b0);
-          // ---- compact the points of the batch that meet the block --------------------
-          bool hit = false;
-          N3Point pt;
-          int rel[3] = {0, 0, 0};
-          if (tid < nb) {
-            pt = p.pts[beg + b0 + tid];
-            hit = true;
+    for (int cy_ = max(0, by - p.nback[1]); cy_ <= by; ++cy_) {
+      const int row = (cz_ * p.nbins[1] + cy_) * p.nbins[0];
+      const int lo = row + max(0, bx - p.nback[0]), hi = row + bx;
+      rbeg[nr] = p.bin_start[lo];
+      rlen[nr] = p.bin_start[hi] + p.bin_count[hi] - rbeg[nr];
+      total_pts += rlen[nr];
+      ++nr;
+    }
+  for (unsigned b0 = 0; b0 < total_pts; b0 += PB) {
+    const int nb = min<unsigned>(PB, total_pts - b0);
+    // ---- compact the points of the batch that meet the block --------------------
+    bool hit = false;
+    N3Point pt;
+    int rel[3] = {0, 0, 0};
+    if (tid < nb) {
+      unsigned v = b0 + tid;
+      int r = 0;
+      while (v >= rlen[r]) v -= rlen[r++];
+      pt = p.pts[rbeg[r] + v];
+      hit = true;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              rel[d] = pt.l0[d] - org[d];
-              hit = hit && rel[d] < B && rel[d] + w > 0;
-            }
-          }
-          const unsigned long long mask = __ballot(hit);
-          __syncthreads();  // previous batch fully consumed
-          if (lane == 0) wave_cnt[wave] = __popcll(mask);
-          __syncthreads();
-          int slot = __popcll(mask & ((1ull << lane) - 1ull));
-          int total = 0;
-          for (int q = 0; q < SP_NT / 64; ++q) {
-            if (q < wave) slot += wave_cnt[q];
-            total += wave_cnt[q];
-          }
-          if (hit) {
-            porg[slot * 4 + 0] = rel[0];
-            porg[slot * 4 + 1] = rel[1];
-            porg[slot * 4 + 2] = rel[2];
-            for (int d = 0; d < 3; ++d)
-              for (int o = 0; o < w; ++o)
-                pw[((size_t)slot * 3 + d) * w + o] =
-                    es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
-            const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
-            const R m = R(pt.mult);
-            const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
-            for (int lv = 0; lv < L; ++lv) {
-              const TV c = vv[(size_t)lv * p.vv_stride + pt.src];
-              pv[((size_t)slot * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
-              pv[((size_t)slot * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
-            }
-          }
-          __syncthreads();
-          n_visit += nb;
-          n_hit += total;
-          // ---- every thread adds the points covering its two cells ----------------------
-          for (int q = 0; q < total; ++q) {
-            const int ly = cy - porg[q * 4 + 1], lz = cz - porg[q * 4 + 2];
-            if ((unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w) {
-              const R wyz = pw[((size_t)q * 3 + 1) * w + ly] * pw[((size_t)q * 3 + 2) * w + lz];
-              const int lx = cx0 - porg[q * 4 + 0];
-              const R w0 = (unsigned)lx < (unsigned)w ? pw[(size_t)q * 3 * w + lx] * wyz : R(0);
-              const R w1 =
-                  (unsigned)(lx + 1) < (unsigned)w ? pw[(size_t)q * 3 * w + lx + 1] * wyz : R(0);
-              const R* v = pv + (size_t)q * L * 2;
+      for (int d = 0; d < 3; ++d) {
+        rel[d] = pt.l0[d] - org[d];
+        hit = hit && rel[d] < B && rel[d] + w > 0;
+      }
+    }
+    const unsigned long long mask = __ballot(hit);
+    __syncthreads();  // previous batch fully consumed
+    if (lane == 0) wave_cnt[wave] = __popcll(mask);
+    __syncthreads();
+    int slot = __popcll(mask & ((1ull << lane) - 1ull));
+    int total = 0;
+    for (int q = 0; q < SP_NT / 64; ++q) {
+      if (q < wave) slot += wave_cnt[q];
+      total += wave_cnt[q];
+    }
+    if (hit) {
+      porg[slot * 4 + 0] = rel[0];
+      porg[slot * 4 + 1] = rel[1];
+      porg[slot * 4 + 2] = rel[2];
+      for (int d = 0; d < 3; ++d)
+        for (int o = 0; o < w; ++o)
+          pw[((size_t)slot * 3 + d) * w + o] =
+              es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
+      const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
+      const R m = R(pt.mult);
+      const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
+      for (int lv = 0; lv < L; ++lv) {
+        const TV c = vv[(size_t)lv * p.vv_stride + pt.src];
+        pv[((size_t)slot * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
+        pv[((size_t)slot * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+      }
+    }
+    __syncthreads();
+    n_visit += nb;
+    n_hit += total;
+    // ---- every thread adds the points covering its two cells ----------------------
+    for (int q = 0; q < total; ++q) {
+      const int ly = cy - porg[q * 4 + 1], lz = cz - porg[q * 4 + 2];
+      if ((unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w) {
+        const R wyz = pw[((size_t)q * 3 + 1) * w + ly] * pw[((size_t)q * 3 + 2) * w + lz];
+        const int lx = cx0 - porg[q * 4 + 0];
+        const R w0 = (unsigned)lx < (unsigned)w ? pw[(size_t)q * 3 * w + lx] * wyz : R(0);
+        const R w1 =
+            (unsigned)(lx + 1) < (unsigned)w ? pw[(size_t)q * 3 * w + lx + 1] * wyz : R(0);
+        const R* v = pv + (size_t)q * L * 2;
 #pragma unroll
-              for (int lv = 0; lv < SP_LMAX; ++lv) {
-                if (lv < L) {
-                  const R vr = v[lv * 2], vi = v[lv * 2 + 1];
-                  acc[0][lv][0] += w0 * vr;
-                  acc[0][lv][1] += w0 * vi;
-                  acc[1][lv][0] += w1 * vr;
-                  acc[1][lv][1] += w1 * vi;
-                }
-              }
-            }
+        for (int lv = 0; lv < SP_LMAX; ++lv) {
+          if (lv < L) {
+            const R vr = v[lv * 2], vi = v[lv * 2 + 1];
+            acc[0][lv][0] += w0 * vr;
+            acc[0][lv][1] += w0 * vi;
+            acc[1][lv][0] += w1 * vr;
+            acc[1][lv][1] += w1 * vi;
           }
         }
       }
+    }
+  }
   if (p.prof && tid == 0) {
     unsigned long long* pr = p.prof + (size_t)bin * 4;
     pr[0] = t_start;
@@ -458,9 +470,9 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int n
 // sub-transform `sub` at axis index sub * sub_stride + i * estride.  The LDS line stride is
 // odd so that the TW lines of a load fall into different banks.
 template <typename R>
-__global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
+__global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p, int tw_shift) {
   extern __shared__ __align__(16) unsigned char fsh[];
-  constexpr int TW = 16;
+  const int TW = 1 << tw_shift;  // x positions per item (power of two)
   const int ld = p.m | 1;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
   cplx<R>* b = a + TW * ld;
@@ -470,45 +482,46 @@ __global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p) {
   cplx<R>* tws2 = tws + twid_len(p.m);
   if (p.twiddle) load_twid<R>(p.tw2, static_cast<int>(p.ntot), tws2, threadIdx.x, blockDim.x);
   const Twid<R> tw2{tws2, tws2 + 64};
-  const int n_xt = (p.xw.count + TW - 1) / TW;
-  const long long n_items = (long long)n_xt * p.n_sub * p.other.count * p.n_levels;
-  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const int xt = static_cast<int>(item % n_xt);
-    long long rest = item / n_xt;
-    const int sub = static_cast<int>(rest % p.n_sub);
+  const int n_xt = (p.xw.count + TW - 1) >> tw_shift;
+  // the launcher checks that the item count fits in 31 bits
+  const int n_items = n_xt * p.n_sub * p.other.count * p.n_levels;
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int xt = item % n_xt;
+    int rest = item / n_xt;
+    const int sub = rest % p.n_sub;
     rest /= p.n_sub;
-    const int o = static_cast<int>(rest % p.other.count);
-    const int lv = static_cast<int>(rest / p.other.count);
+    const int o = rest % p.other.count;
+    const int lv = rest / p.other.count;
     int oo = p.other.start + o;
     oo -= oo >= p.no ? p.no : 0;
     if (p.operm_n2 > 1) oo = (oo % p.operm_n1) * p.operm_n2 + oo / p.operm_n1;
     cplx<R>* base = p.grid + (size_t)lv * p.grid_stride + (size_t)oo * p.outer_stride;
-    for (int t = threadIdx.x; t < TW * p.m; t += blockDim.x) {
-      const int xi = t % TW, i = t / TW;
-      const int j = xt * TW + xi;
+    for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
+      const int xi = t & (TW - 1), i = t >> tw_shift;
+      const int j = (xt << tw_shift) + xi;
       cplx<R> v = mk<R>(0, 0);
       if (j < p.xw.count) {
         int x = p.xw.start + j;
         x -= x >= p.nx ? p.nx : 0;
-        const long long ai = (long long)sub * p.sub_stride + (long long)i * p.estride;
-        if (!p.in_check || in_window(static_cast<int>(ai), p.in, p.nax))
-          v = base[ai * p.axis_stride + x];
+        const int ai = static_cast<int>(sub * p.sub_stride + i * p.estride);
+        if (!p.in_check || in_window(ai, p.in, p.nax))
+          v = base[(long long)ai * p.axis_stride + x];
       }
       a[xi * ld + i] = v;
     }
     __syncthreads();
     cplx<R>* res = lds_fft<R>(a, b, p.m, TW, ld, p.rad, tw, threadIdx.x, blockDim.x);
-    for (int t = threadIdx.x; t < TW * p.m; t += blockDim.x) {
-      const int xi = t % TW, k = t / TW;
-      const int j = xt * TW + xi;
+    for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
+      const int xi = t & (TW - 1), k = t >> tw_shift;
+      const int j = (xt << tw_shift) + xi;
       if (j >= p.xw.count) continue;
       if (p.out_check && !in_window(sub * p.fa + k * p.fb, p.out, p.nax)) continue;
       int x = p.xw.start + j;
       x -= x >= p.nx ? p.nx : 0;
       cplx<R> v = res[xi * ld + k];
-      if (p.twiddle) v = v * tw2(static_cast<int>(((long long)sub * k) % p.ntot));
-      const long long ai = (long long)sub * p.sub_stride + (long long)k * p.estride;
-      base[ai * p.axis_stride + x] = v;
+      if (p.twiddle) v = v * tw2(sub * k);  // sub < n2, k < n1: sub k < ntot
+      const int ai = static_cast<int>(sub * p.sub_stride + k * p.estride);
+      base[(long long)ai * p.axis_stride + x] = v;
     }
     __syncthreads();
   }
@@ -623,6 +636,8 @@ __global__ __launch_bounds__(256) void vv_levels_kernel(N3VVParams<R> p) {
   for (int e0 = 0; e0 < p.n_cols; e0 += 16) {
     const int e = e0 + ty;
     const R de = e < p.n_cols ? dw[e] : R(0);
+    // eigenvectors outside the level carry weight 0: skip chunks without any (uniform)
+    if (!__syncthreads_or(de != R(0))) continue;
     const int i = i0 + tx, j = j0 + tx;
     ti[ty][tx] = (e < p.n_cols && i < n) ? Z[i + (size_t)e * p.ldz] * de : mk<R>(0, 0);
     tj[ty][tx] = (e < p.n_cols && j < n) ? Z[j + (size_t)e * p.ldz] : mk<R>(0, 0);
@@ -730,7 +745,7 @@ void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
   if (n_blocks > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT spread blocks");
   for (int d = 0; d < 3; ++d)
     if (p_all.bin_size[d] != SP_BIN || (long long)p_all.nbins[d] * SP_BIN < p_all.nf1[d] ||
-        p_all.nback[d] < 0)
+        p_all.nback[d] < 0 || p_all.nback[d] > 2)
       fail(BIPP_INTERNAL_ERROR, "NUFFT spread blocks do not cover the grid");
   static bool attr_set = false;
   if (!attr_set) {
@@ -790,13 +805,20 @@ template void launch_fft_lines<double>(const N3LineParams<double>&, hipStream_t)
 
 template <typename R>
 void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
-  const long long items =
-      (long long)((p.xw.count + 15) / 16) * p.n_sub * p.other.count * p.n_levels;
-  if (items <= 0) return;
   if (!window_ok(p.xw, p.nx) || !window_ok(p.other, p.no) || !window_ok(p.in, p.nax) ||
-      !window_ok(p.out, p.nax) || (long long)p.m * p.n_sub != p.nax)
+      !window_ok(p.out, p.nax) || (long long)p.m * p.n_sub != p.nax ||
+      (p.twiddle && p.ntot != p.nax))
     fail(BIPP_INTERNAL_ERROR, "FFT pass windows out of range");
-  const size_t sh = sizeof(cplx<R>) * (2 * 16 * (size_t)(p.m | 1) + twid_len(p.m) +
+  // x positions per item: a power of two in [16, 64] with about 4096 elements per item
+  const int ld = p.m | 1;
+  int tw_shift = 4;
+  while (tw_shift < 6 && (ld << (tw_shift + 1)) <= 4096) ++tw_shift;
+  const int TW = 1 << tw_shift;
+  const long long items =
+      (long long)((p.xw.count + TW - 1) / TW) * p.n_sub * p.other.count * p.n_levels;
+  if (items <= 0) return;
+  if (items > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT grid too large for one pass");
+  const size_t sh = sizeof(cplx<R>) * (2 * (size_t)TW * ld + twid_len(p.m) +
                                        (p.twiddle ? twid_len(static_cast<int>(p.ntot)) : 0));
   if (sh > 160 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");
   static bool attr_set = false;
@@ -806,7 +828,7 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
     attr_set = true;
   }
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 32768));
-  fft_pass_kernel<R><<<blocks, 256, sh, st>>>(p);
+  fft_pass_kernel<R><<<blocks, 256, sh, st>>>(p, tw_shift);
   check_launch("fft_pass_kernel");
 }
 template void launch_fft_pass<float>(const N3PassParams<float>&, hipStream_t);

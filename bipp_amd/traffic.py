@@ -39,6 +39,12 @@ def mean_for(rows, needle: str):
     return (sum(sel) / len(sel), len(sel)) if sel else (None, 0)
 
 
+def largest_for(rows, needle: str):
+    """The calibration copy is by far the largest dispatch of the copy kernel."""
+    sel = [v for n, v in rows if needle in n]
+    return max(sel) if sel else None
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--fetch", required=True)
@@ -53,8 +59,8 @@ def main() -> None:
     w_rows = per_dispatch(a.write, "WRITE_SIZE")
     fk, nf = mean_for(f_rows, a.kernel)
     wk, nw = mean_for(w_rows, a.kernel)
-    fc, _ = mean_for(f_rows, a.copy_kernel)
-    wc, _ = mean_for(w_rows, a.copy_kernel)
+    fc = largest_for(f_rows, a.copy_kernel)
+    wc = largest_for(w_rows, a.copy_kernel)
     if fk is None or wk is None or not fc or not wc:
         raise SystemExit("kernel or copy dispatches missing from the counter files")
     expect_kib = a.copy_bytes / 1024.0
