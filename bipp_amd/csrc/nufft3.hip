@@ -30,10 +30,19 @@ namespace bippamd {
 
 namespace {
 
-constexpr int SP_NT = 256;   // spread workgroup: 2 cells of an 8^3 block per thread
-constexpr int SP_BIN = 8;    // spread block edge
-constexpr int SP_LMAX = 8;   // levels per spread launch (sums kept in registers)
+constexpr int SP_BIN = 8;    // spread block edge: one wavefront per 8^3 block
+constexpr int SP_LG = 4;     // levels per spread launch (sums kept in registers)
+constexpr int SP_WMAX = 16;  // largest kernel width
 constexpr int SP_MAXR = 9;   // neighbour bin ranges per block: (nback + 1)^2 with nback <= 2
+constexpr int FFT_NT = 512;  // FFT workgroup: ~2 radix-4 butterflies per thread per stage
+
+// Orders the LDS accesses of the lanes of one wavefront around a hand-off through LDS
+// (lanes write, other lanes of the same wavefront read afterwards, no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename R>
 __device__ __forceinline__ R es_kernel(R z, R beta) {
@@ -152,40 +161,58 @@ __global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, c
 }
 
 // --------------------------------------------------------------------------------------
-// Spreading with output ownership.  The workgroup of block (bx, by, bz) owns the 8^3 cells
-// l + nf1/2 in [8 b, 8 b + 8) per axis; thread t owns the two cells
-// (2 (t mod 4) + {0, 1}, (t / 4) mod 8, t / 32) and keeps their sums for every level in
-// registers.  The workgroup walks the bins that can reach the block (a point of bin c covers
-// cells [l0, l0 + w), l0 in bin c), a batch of PB points at a time: each point that meets
-// the block is compacted into LDS with its kernel weights per axis and its values, then
-// every thread adds the points that cover its cells.  The block is written once, times the
-// deconvolution factors, with plain stores: no atomics, a fixed summation order, and every
-// cell of [-nf1/2, nf1/2)^3 written by exactly one workgroup (no zero pass over the grid).
+// Spreading with output ownership.  One workgroup owns an 8^3 block (cells l + nf1/2 in
+// [8 b, 8 b + 8) per axis); in each of its wavefronts lane t owns the x-row
+// (y = t mod 8, z = t / 8) and keeps its 8 cells for SP_LG levels in registers.  The points of
+// the bins that can reach the block (a point of bin c covers cells [l0, l0 + w), l0 in bin c)
+// are taken 64 at a time, batch k by wavefront k mod (wavefronts): each point that meets the block
+// is compacted into the wavefront's LDS buffers with its kernel weights (the x weights
+// expanded to the block's 8 columns, zero outside the support) and its values; then every
+// lane whose row lies inside a point's y/z support adds it to its 8 cells.  The wavefronts'
+// sums are reduced through LDS and the block is written once, times the deconvolution
+// factors, with plain stores: no atomics, a fixed summation order, and every cell of
+// [-nf1/2, nf1/2)^3 written by exactly one workgroup (no zero pass over the grid).
 // --------------------------------------------------------------------------------------
+template <typename R>
+struct SpreadBatch {
+  int porg[64][2];        // y, z of l0 - block origin
+  R pwx[64][SP_BIN];      // x weight of each block column
+  R pwy[64][SP_WMAX];     // kernel weights along y, z by kernel offset
+  R pwz[64][SP_WMAX];
+  R pv[64][SP_LG][2];     // values of the levels of this launch
+};
+
+// wavefronts per spread block (batches split between them); static LDS stays under 64 KiB
+template <typename R>
+struct SpreadWaves {
+  static constexpr int value = sizeof(R) == 8 ? 2 : 4;
+};
+
 template <typename R, typename TV>
-__global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, int PB) {
-  extern __shared__ __align__(16) unsigned char sp_sh[];
-  constexpr int B = SP_BIN;
+__global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
+  constexpr int B = SP_BIN, PB = 64, NWV = SpreadWaves<R>::value;
+  // per-wavefront batch buffers; reused at the end for the reduction of the row sums
+  __shared__ union {
+    SpreadBatch<R> batch[NWV];
+    R red[NWV - 1][64][B * SP_LG * 2];
+  } sh;
   const int w = p.w;
   const int L = p.n_levels;
-  int* porg = reinterpret_cast<int*>(sp_sh);                     // [PB][4] l0 - block origin
-  R* pw = reinterpret_cast<R*>(porg + (size_t)PB * 4);           // [PB][3][w] kernel weights
-  R* pv = pw + (size_t)PB * 3 * w;                               // [PB][L][2] values
-  __shared__ int wave_cnt[SP_NT / 64 + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  SpreadBatch<R>& sb = sh.batch[wave];
   const int bin = blockIdx.x;
   const int bx = bin % p.nbins[0];
   const int by = (bin / p.nbins[0]) % p.nbins[1];
   const int bz = bin / (p.nbins[0] * p.nbins[1]);
   const int org[3] = {bx * B, by * B, bz * B};
-  const int cx0 = 2 * (tid & 3), cy = (tid >> 2) & 7, cz = tid >> 5;
-  R acc[2][SP_LMAX][2];
+  const int cy = lane & 7, cz = lane >> 3;
+  R acc[B][SP_LG][2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+  for (int c = 0; c < B; ++c)
 #pragma unroll
-    for (int lv = 0; lv < SP_LMAX; ++lv) acc[c][lv][0] = acc[c][lv][1] = R(0);
+    for (int lv = 0; lv < SP_LG; ++lv) acc[c][lv][0] = acc[c][lv][1] = R(0);
   const unsigned long long t_start = p.prof ? wall_clock64() : 0;
-  unsigned long long n_visit = 0, n_hit = 0;
+  unsigned long long n_hit = 0;
   const R two_over_w = R(2) / R(w);
   const R beta = R(p.beta);
   // The bins that can reach the block: for each (cy, cz) of the neighbourhood the bins
@@ -203,14 +230,18 @@ __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, i
       total_pts += rlen[nr];
       ++nr;
     }
-  for (unsigned b0 = 0; b0 < total_pts; b0 += PB) {
+  const unsigned n_batches = (total_pts + PB - 1) / PB;
+  // wavefronts that own at least one batch (the others reach the barriers below idle)
+  const int n_active = static_cast<int>(min<unsigned>(n_batches, NWV));
+  for (unsigned bi = wave; bi < n_batches; bi += NWV) {
+    const unsigned b0 = bi * PB;
     const int nb = min<unsigned>(PB, total_pts - b0);
-    // ---- compact the points of the batch that meet the block --------------------
     bool hit = false;
     N3Point pt;
     int rel[3] = {0, 0, 0};
-    if (tid < nb) {
-      unsigned v = b0 + tid;
+    TV vals[SP_LG];
+    if (lane < nb) {
+      unsigned v = b0 + lane;
       int r = 0;
       while (v >= rlen[r]) v -= rlen[r++];
       pt = p.pts[rbeg[r] + v];
@@ -220,66 +251,113 @@ __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, i
         rel[d] = pt.l0[d] - org[d];
         hit = hit && rel[d] < B && rel[d] + w > 0;
       }
-    }
-    const unsigned long long mask = __ballot(hit);
-    __syncthreads();  // previous batch fully consumed
-    if (lane == 0) wave_cnt[wave] = __popcll(mask);
-    __syncthreads();
-    int slot = __popcll(mask & ((1ull << lane) - 1ull));
-    int total = 0;
-    for (int q = 0; q < SP_NT / 64; ++q) {
-      if (q < wave) slot += wave_cnt[q];
-      total += wave_cnt[q];
-    }
-    if (hit) {
-      porg[slot * 4 + 0] = rel[0];
-      porg[slot * 4 + 1] = rel[1];
-      porg[slot * 4 + 2] = rel[2];
-      for (int d = 0; d < 3; ++d)
-        for (int o = 0; o < w; ++o)
-          pw[((size_t)slot * 3 + d) * w + o] =
-              es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta);
-      const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
-      const R m = R(pt.mult);
-      const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
-      for (int lv = 0; lv < L; ++lv) {
-        const TV c = vv[(size_t)lv * p.vv_stride + pt.src];
-        pv[((size_t)slot * L + lv) * 2 + 0] = m * (R(c.re) * pr - R(c.im) * pim);
-        pv[((size_t)slot * L + lv) * 2 + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+      if (hit) {
+        // values first: their latency overlaps the compaction and the kernel weights
+        const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
+#pragma unroll
+        for (int lv = 0; lv < SP_LG; ++lv)
+          if (lv < L) vals[lv] = vv[(size_t)lv * p.vv_stride + pt.src];
       }
     }
-    __syncthreads();
-    n_visit += nb;
-    n_hit += total;
-    // ---- every thread adds the points covering its two cells ----------------------
-    for (int q = 0; q < total; ++q) {
-      const int ly = cy - porg[q * 4 + 1], lz = cz - porg[q * 4 + 2];
-      if ((unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w) {
-        const R wyz = pw[((size_t)q * 3 + 1) * w + ly] * pw[((size_t)q * 3 + 2) * w + lz];
-        const int lx = cx0 - porg[q * 4 + 0];
-        const R w0 = (unsigned)lx < (unsigned)w ? pw[(size_t)q * 3 * w + lx] * wyz : R(0);
-        const R w1 =
-            (unsigned)(lx + 1) < (unsigned)w ? pw[(size_t)q * 3 * w + lx + 1] * wyz : R(0);
-        const R* v = pv + (size_t)q * L * 2;
+    const unsigned long long mask = __ballot(hit);
+    const int total = __popcll(mask);
+    const int slot = __popcll(mask & ((1ull << lane) - 1ull));
+    wave_sync();  // the previous batch of this wavefront fully consumed
+    if (hit) {
+      sb.porg[slot][0] = rel[1];
+      sb.porg[slot][1] = rel[2];
 #pragma unroll
-        for (int lv = 0; lv < SP_LMAX; ++lv) {
-          if (lv < L) {
-            const R vr = v[lv * 2], vi = v[lv * 2 + 1];
-            acc[0][lv][0] += w0 * vr;
-            acc[0][lv][1] += w0 * vi;
-            acc[1][lv][0] += w1 * vr;
-            acc[1][lv][1] += w1 * vi;
-          }
+      for (int cx = 0; cx < B; ++cx) {
+        const int o = cx - rel[0];
+        sb.pwx[slot][cx] = (o >= 0 && o < w)
+                               ? es_kernel<R>((R(o) - R(pt.fr[0])) * two_over_w, beta)
+                               : R(0);
+      }
+      for (int o = 0; o < w; ++o) {
+        sb.pwy[slot][o] = es_kernel<R>((R(o) - R(pt.fr[1])) * two_over_w, beta);
+        sb.pwz[slot][o] = es_kernel<R>((R(o) - R(pt.fr[2])) * two_over_w, beta);
+      }
+      const R m = R(pt.mult);
+      const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
+#pragma unroll
+      for (int lv = 0; lv < SP_LG; ++lv) {
+        if (lv < L) {
+          const TV c = vals[lv];
+          sb.pv[slot][lv][0] = m * (R(c.re) * pr - R(c.im) * pim);
+          sb.pv[slot][lv][1] = m * (R(c.re) * pim + R(c.im) * pr);
+        } else {
+          sb.pv[slot][lv][0] = sb.pv[slot][lv][1] = R(0);
         }
       }
     }
+    wave_sync();
+    n_hit += total;
+    // Branch-free: rows outside a point's support take weight 0 (the weight reads use
+    // clamped, in-range offsets); two points per iteration keep two load chains in flight.
+    auto row_weight = [&](int q) {
+      const int ly = cy - sb.porg[q][0], lz = cz - sb.porg[q][1];
+      const bool in = (unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w;
+      const int lyc = min(max(ly, 0), w - 1), lzc = min(max(lz, 0), w - 1);
+      const R wyz = sb.pwy[q][lyc] * sb.pwz[q][lzc];
+      return in ? wyz : R(0);
+    };
+    auto add_point = [&](int q, R wyz) {
+      R vr[SP_LG], vi[SP_LG];
+#pragma unroll
+      for (int lv = 0; lv < SP_LG; ++lv) {
+        vr[lv] = sb.pv[q][lv][0] * wyz;
+        vi[lv] = sb.pv[q][lv][1] * wyz;
+      }
+#pragma unroll
+      for (int cx = 0; cx < B; ++cx) {
+        const R wx = sb.pwx[q][cx];
+#pragma unroll
+        for (int lv = 0; lv < SP_LG; ++lv) {
+          acc[cx][lv][0] += wx * vr[lv];
+          acc[cx][lv][1] += wx * vi[lv];
+        }
+      }
+    };
+    int q = 0;
+    for (; q + 1 < total; q += 2) {
+      const R w0 = row_weight(q), w1 = row_weight(q + 1);
+      add_point(q, w0);
+      add_point(q + 1, w1);
+    }
+    if (q < total) add_point(q, row_weight(q));
   }
-  if (p.prof && tid == 0) {
+  // ---- reduce the wavefronts' row sums into wavefront 0 --------------------------------
+  if (n_active > 1) {
+    __syncthreads();  // every wavefront is done with its batch buffers
+    if (wave > 0 && wave < n_active) {
+      R* dst = sh.red[wave - 1][lane];
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+#pragma unroll
+        for (int lv = 0; lv < SP_LG; ++lv) {
+          dst[(c * SP_LG + lv) * 2 + 0] = acc[c][lv][0];
+          dst[(c * SP_LG + lv) * 2 + 1] = acc[c][lv][1];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < n_active - 1 && wave == 0; ++k) {
+      const R* src = sh.red[k][lane];
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+#pragma unroll
+        for (int lv = 0; lv < SP_LG; ++lv) {
+          acc[c][lv][0] += src[(c * SP_LG + lv) * 2 + 0];
+          acc[c][lv][1] += src[(c * SP_LG + lv) * 2 + 1];
+        }
+    }
+  }
+  if (wave > 0) return;  // no barrier below: wavefront 0 writes the block
+  if (p.prof && lane == 0) {
     unsigned long long* pr = p.prof + (size_t)bin * 4;
     pr[0] = t_start;
     pr[1] = wall_clock64();
-    pr[2] = n_visit;
-    pr[3] = n_hit;
+    pr[2] = total_pts;
+    pr[3] = n_hit;  // hits of wavefront 0
   }
   // ---- deconvolve and store the owned cells inside [-nf1/2, nf1/2)^3 ------------------
   const int iy = org[1] + cy, iz = org[2] + cz;
@@ -289,16 +367,16 @@ __global__ __launch_bounds__(SP_NT) void n3_spread_kernel(N3SpreadParams<R> p, i
   gz += gz < 0 ? p.nf2[2] : 0;
   const R fyz = p.fy[iy] * p.fz[iz];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int ix = org[0] + cx0 + c;
+  for (int cx = 0; cx < B; ++cx) {
+    const int ix = org[0] + cx;
     if (ix >= p.nf1[0]) continue;
     const R f = p.fx[ix] * fyz;
     int gx = ix - p.nf1[0] / 2;
     gx += gx < 0 ? p.nf2[0] : 0;
     cplx<R>* g = p.grid + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx;
 #pragma unroll
-    for (int lv = 0; lv < SP_LMAX; ++lv)
-      if (lv < L) g[(size_t)(p.lv0 + lv) * p.grid_stride] = mk<R>(acc[c][lv][0] * f, acc[c][lv][1] * f);
+    for (int lv = 0; lv < SP_LG; ++lv)
+      if (lv < L) g[(size_t)(p.lv0 + lv) * p.grid_stride] = mk<R>(acc[cx][lv][0] * f, acc[cx][lv][1] * f);
   }
 }
 
@@ -426,22 +504,28 @@ __device__ __forceinline__ bool in_window(int a, N3Window wd, int n) {
   return d < wd.count;
 }
 
-// Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).
+// Elements per thread of one FFT item: items hold at most 4096 elements (the launchers
+// check), so 8 per thread of an FFT_NT = 512 workgroup.
+constexpr int FFT_PF = 8;
+
+// Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).  The
+// next group's inputs are fetched into registers while the current group is transformed.
 template <typename R>
-__global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int nl) {
+__global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, int nl) {
   extern __shared__ __align__(16) unsigned char fsh[];
-  __shared__ long long lbase[16];
+  __shared__ long long lbase[2][16];
   const int n = p.n;
+  const int tid = threadIdx.x;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
   cplx<R>* b = a + (size_t)nl * n;
-  const Twid<R> tw = load_twid<R>(p.tw, n, b + (size_t)nl * n, threadIdx.x, blockDim.x);
+  const Twid<R> tw = load_twid<R>(p.tw, n, b + (size_t)nl * n, tid, FFT_NT);
   const long long n_lines = (long long)p.n_levels * p.zw.count * p.yw.count;
   const long long n_groups = (n_lines + nl - 1) / nl;
-  for (long long grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+  auto set_bases = [&](long long grp, int buf) {
     const long long line0 = grp * nl;
     const int nlg = static_cast<int>(min<long long>(nl, n_lines - line0));
-    if ((int)threadIdx.x < nlg) {
-      const long long line = line0 + threadIdx.x;
+    if (tid < nlg) {
+      const long long line = line0 + tid;
       const int yo = static_cast<int>(line % p.yw.count);
       const long long r = line / p.yw.count;
       const int zo = static_cast<int>(r % p.zw.count);
@@ -449,28 +533,56 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(N3LineParams<R> p, int n
       int y = p.yw.start + yo, z = p.zw.start + zo;
       y -= y >= p.ny ? p.ny : 0;
       z -= z >= p.nz ? p.nz : 0;
-      lbase[threadIdx.x] = (long long)lv * p.grid_stride + ((long long)z * p.ny + y) * n;
+      lbase[buf][tid] = (long long)lv * p.grid_stride + ((long long)z * p.ny + y) * n;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nlg * n; i += blockDim.x) {
-      const int li = i / n, x = i - li * n;
-      a[i] = in_window(x, p.in, n) ? p.grid[lbase[li] + x] : mk<R>(0, 0);
+  };
+  auto fetch = [&](long long grp, int buf, cplx<R>* regs) {
+    const int nlg = static_cast<int>(min<long long>(nl, n_lines - grp * nl));
+#pragma unroll
+    for (int k = 0; k < FFT_PF; ++k) {
+      const int i = tid + k * FFT_NT;
+      if (i < nlg * n) {
+        const int li = i / n, x = i - li * n;
+        regs[k] = in_window(x, p.in, n) ? p.grid[lbase[buf][li] + x] : mk<R>(0, 0);
+      }
     }
+  };
+  long long grp = blockIdx.x;
+  if (grp >= n_groups) return;
+  cplx<R> cur[FFT_PF], nxt[FFT_PF];
+  int cb = 0;
+  set_bases(grp, cb);
+  __syncthreads();
+  fetch(grp, cb, cur);
+  for (; grp < n_groups; grp += gridDim.x) {
+    const int nlg = static_cast<int>(min<long long>(nl, n_lines - grp * nl));
+    const long long gn = grp + gridDim.x;
+#pragma unroll
+    for (int k = 0; k < FFT_PF; ++k) {
+      const int i = tid + k * FFT_NT;
+      if (i < nlg * n) a[i] = cur[k];
+    }
+    if (gn < n_groups) set_bases(gn, cb ^ 1);
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, tw, threadIdx.x, blockDim.x);
-    for (int i = threadIdx.x; i < nlg * n; i += blockDim.x) {
+    if (gn < n_groups) fetch(gn, cb ^ 1, nxt);  // in flight during the transform
+    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, tw, tid, FFT_NT);
+    for (int i = tid; i < nlg * n; i += FFT_NT) {
       const int li = i / n, k = i - li * n;
-      if (in_window(k, p.out, n)) p.grid[lbase[li] + k] = res[i];
+      if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
     }
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < FFT_PF; ++k) cur[k] = nxt[k];
+    cb ^= 1;
   }
 }
 
 // Strided axis: transforms of length m for TW consecutive x positions at once; element i of
 // sub-transform `sub` at axis index sub * sub_stride + i * estride.  The LDS line stride is
-// odd so that the TW lines of a load fall into different banks.
+// odd so that the TW lines of a load fall into different banks.  (Latency is hidden by
+// occupancy: a register prefetch of the next item costs more waves than it saves.)
 template <typename R>
-__global__ __launch_bounds__(256) void fft_pass_kernel(N3PassParams<R> p, int tw_shift) {
+__global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int tw_shift) {
   extern __shared__ __align__(16) unsigned char fsh[];
   const int TW = 1 << tw_shift;  // x positions per item (power of two)
   const int ld = p.m | 1;
@@ -733,38 +845,23 @@ void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* b
   check_launch("n3_scatter_kernel");
 }
 
-template <typename R>
-size_t n3_spread_shmem(int pb, int w, int n_levels) {
-  return sizeof(int) * 4 * (size_t)pb + sizeof(R) * (size_t)pb * (3 * w + 2 * n_levels);
-}
-
 template <typename R, typename TV>
 void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
   const long long n_blocks = (long long)p_all.nbins[0] * p_all.nbins[1] * p_all.nbins[2];
   if (n_blocks <= 0) return;
   if (n_blocks > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT spread blocks");
+  if (p_all.w < 1 || p_all.w > SP_WMAX) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT kernel width above 16");
   for (int d = 0; d < 3; ++d)
     if (p_all.bin_size[d] != SP_BIN || (long long)p_all.nbins[d] * SP_BIN < p_all.nf1[d] ||
         p_all.nback[d] < 0 || p_all.nback[d] > 2)
       fail(BIPP_INTERNAL_ERROR, "NUFFT spread blocks do not cover the grid");
-  static bool attr_set = false;
-  if (!attr_set) {
-    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&n3_spread_kernel<R, TV>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    attr_set = true;
-  }
+  // SP_LG levels per launch (the sums of a lane's 8 cells live in registers)
   const int L = p_all.n_levels_total;
-  const int group = std::min(L, SP_LMAX);
-  // points per batch: as many as the LDS budget allows (one per thread at most)
-  int pb = SP_NT;
-  while (pb > 64 && n3_spread_shmem<R>(pb, p_all.w, group) > 64 * 1024) pb /= 2;
-  for (int lv0 = 0; lv0 < L; lv0 += group) {
+  for (int lv0 = 0; lv0 < L; lv0 += SP_LG) {
     N3SpreadParams<R> p = p_all;
     p.lv0 = lv0;
-    p.n_levels = std::min(group, L - lv0);
-    const size_t sh = n3_spread_shmem<R>(pb, p.w, p.n_levels);
-    if (sh > 150 * 1024) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT spread batch exceeds LDS");
-    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), SP_NT, sh, st>>>(p, pb);
+    p.n_levels = std::min(SP_LG, L - lv0);
+    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), 64 * SpreadWaves<R>::value, 0, st>>>(p);
     check_launch("n3_spread_kernel");
   }
 }
@@ -786,6 +883,7 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
     fail(BIPP_INTERNAL_ERROR, "FFT line windows out of range");
   // several short lines per workgroup so that every stage keeps most lanes busy
   const int nl = std::max(1, std::min(16, 2048 / p.n));
+  if ((long long)nl * p.n > FFT_NT * FFT_PF) fail(BIPP_INTERNAL_ERROR, "FFT line group too long");
   const size_t sh = sizeof(cplx<R>) * (2 * (size_t)p.n * nl + twid_len(p.n));
   // 1 KiB of the 160 KiB stays for the kernel's static line bases
   if (sh > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
@@ -797,7 +895,7 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
   }
   const long long groups = (n_lines + nl - 1) / nl;
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(groups, 32768));
-  fft_lines_kernel<R><<<blocks, 256, sh, st>>>(p, nl);
+  fft_lines_kernel<R><<<blocks, FFT_NT, sh, st>>>(p, nl);
   check_launch("fft_lines_kernel");
 }
 template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
@@ -814,6 +912,7 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
   int tw_shift = 4;
   while (tw_shift < 6 && (ld << (tw_shift + 1)) <= 4096) ++tw_shift;
   const int TW = 1 << tw_shift;
+  if ((long long)p.m * TW > FFT_NT * FFT_PF) fail(BIPP_INTERNAL_ERROR, "FFT pass item too large");
   const long long items =
       (long long)((p.xw.count + TW - 1) / TW) * p.n_sub * p.other.count * p.n_levels;
   if (items <= 0) return;
@@ -828,7 +927,7 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
     attr_set = true;
   }
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 32768));
-  fft_pass_kernel<R><<<blocks, 256, sh, st>>>(p, tw_shift);
+  fft_pass_kernel<R><<<blocks, FFT_NT, sh, st>>>(p, tw_shift);
   check_launch("fft_pass_kernel");
 }
 template void launch_fft_pass<float>(const N3PassParams<float>&, hipStream_t);
