@@ -111,6 +111,14 @@ __device__ __forceinline__ R block_sum(R v, R* scratch) {
   return s;
 }
 
+// Orders the LDS accesses of the lanes of one wavefront around a hand-off through LDS
+// (lanes write, other lanes of the same wavefront read afterwards, no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
 }  // namespace bippamd

@@ -70,13 +70,26 @@ struct QlState {
   int l, m, iter, in_iter, fail;
 };
 
+// In-workgroup GEMM tiling: GT x GT output tiles, k taken GK at a time through LDS.
+constexpr int GT = 64;
+constexpr int GK = 16;
+
+template <typename R>
+struct GemmTiles {
+  cplx<R> a[GK][GT + 1];  // a[k][r] = op(A)[i0 + r, k0 + k]
+  cplx<R> b[GK][GT + 1];  // b[k][c] = op(B)[k0 + k, j0 + c]
+};
+
 template <typename R, int NMAX>
 struct EigShared {
   static constexpr int NW_ = EigCfg<NMAX>::NW;
   cplx<R> v[NMAX];
   cplx<R> w[NMAX];
   cplx<R> vn[NMAX];
-  cplx<R> xw[NW_][NMAX];
+  union {
+    cplx<R> xw[NW_][NMAX];  // per-wave partial sums of the matrix-vector passes
+    GemmTiles<R> gt;        // operand tiles of the GEMM passes
+  };
   R dd[NMAX];
   R ee[NMAX];
   R cs[2][NMAX];
@@ -129,44 +142,129 @@ __device__ __forceinline__ void cols_update(int j0, int n, int rhi, RowLo rlo,
 }
 
 // --------------------------------------------------------------------------------------
-// Forward substitution B <- L^-1 B on all n columns of B (column-major, ld).
+// W = L^-1 in place on the lower triangle of L (ztrti2, lower, non-unit): columns from the
+// last to the first, W[j+1:n, j] = -W[j+1:n, j+1:n] L[j+1:n, j] / L[j, j], the product
+// taken over the lower triangle only (the strict upper triangle of L is never read).
 // --------------------------------------------------------------------------------------
 template <typename R, int NMAX>
-__device__ void fwd_solve(int n, const cplx<R>* __restrict__ L, int ldl, cplx<R>* __restrict__ B,
-                          int ldb, EigShared<R, NMAX>& sh) {
-  constexpr int NT_ = EigCfg<NMAX>::NT;
-  const int tid = threadIdx.x;
-  for (int k = 0; k < n; ++k) {
-    const R linv = R(1) / L[k + (size_t)k * ldl].re;
-    for (int j = tid; j < n; j += NT_) {
-      cplx<R> r = B[k + (size_t)j * ldb] * linv;
-      B[k + (size_t)j * ldb] = r;
-      sh.w[j] = r;
-    }
-    for (int i = k + 1 + tid; i < n; i += NT_) sh.v[i] = L[i + (size_t)k * ldl];
+__device__ void tri_inverse(int n, cplx<R>* __restrict__ L, int ld, EigShared<R, NMAX>& sh) {
+  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
+                UC = EigCfg<NMAX>::UC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = n - 1; j >= 0; --j) {
+    const int j0 = j + 1;
+    const R wjj = R(1) / L[j + (size_t)j * ld].re;  // read before tid 0 overwrites it below
+    for (int i = j0 + tid; i < n; i += NT_) sh.v[i] = L[i + (size_t)j * ld];
     __syncthreads();
-    cols_update<R, NMAX, false>(0, n, n, [k](int) { return k + 1; }, B, ldb, sh.v, sh.w);
+    cplx<R> acc[RM];
+#pragma unroll
+    for (int r = 0; r < RM; ++r) acc[r] = mk<R>(0, 0);
+    for (int kb = j0 + wave; kb < n; kb += NW_ * UC) {
+      cplx<R> a[UC][RM];
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int k = kb + u * NW_;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          const int i = j0 + lane + 64 * r;
+          if (k < n && i < n && i >= k) a[u][r] = L[i + (size_t)k * ld];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int k = kb + u * NW_;
+        if (k >= n) continue;
+        const cplx<R> vk = sh.v[k];
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          const int i = j0 + lane + 64 * r;
+          if (i < n && i >= k) acc[r] = acc[r] + a[u][r] * vk;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      const int i = j0 + lane + 64 * r;
+      if (i < n) sh.xw[wave][i] = acc[r];
+    }
+    __syncthreads();
+    for (int i = j0 + tid; i < n; i += NT_) {
+      cplx<R> sum = sh.xw[0][i];
+#pragma unroll
+      for (int w = 1; w < NW_; ++w) sum = sum + sh.xw[w][i];
+      L[i + (size_t)j * ld] = sum * (-wjj);
+    }
+    if (tid == 0) L[j + (size_t)j * ld] = mk<R>(wjj, 0);
     __syncthreads();
   }
 }
 
-// Backward substitution B <- L^-H B.
-template <typename R, int NMAX>
-__device__ void bwd_solve_h(int n, const cplx<R>* __restrict__ L, int ldl, cplx<R>* __restrict__ B,
-                            int ldb, EigShared<R, NMAX>& sh) {
-  constexpr int NT_ = EigCfg<NMAX>::NT;
-  const int tid = threadIdx.x;
-  for (int k = n - 1; k >= 0; --k) {
-    const R linv = R(1) / L[k + (size_t)k * ldl].re;
-    for (int j = tid; j < n; j += NT_) {
-      cplx<R> r = B[k + (size_t)j * ldb] * linv;
-      B[k + (size_t)j * ldb] = r;
-      sh.w[j] = r;
+// --------------------------------------------------------------------------------------
+// C = op(A) op(B) for n x n matrices, GT x GT output tiles (with LOWER_ONLY the tiles on
+// and below the diagonal).  fa(i, k) and fb(k, j) return operand elements, zero outside
+// their nonzero pattern; krange(i0, j0, klo, khi) bounds the k that can contribute to a
+// tile.  A_KFAST / B_KFAST: consecutive k are contiguous in memory (else consecutive i / j),
+// which sets how the threads split a tile load.  Lanes own tile rows, wavefronts GT/NW
+// tile columns; the loads of the next k chunk are in flight while the current one is used.
+// Every thread executes the same sequence of barriers.
+// --------------------------------------------------------------------------------------
+template <typename R, int NMAX, bool A_KFAST, bool B_KFAST, bool LOWER_ONLY, typename FA,
+          typename FB, typename KR>
+__device__ void gemm_tiles(int n, FA fa, FB fb, KR krange, cplx<R>* __restrict__ C, int ldc,
+                           EigShared<R, NMAX>& sh) {
+  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW;
+  constexpr int CPW = GT / NW_;
+  constexpr int PER = GT * GK / NT_;
+  static_assert(GT * GK % NT_ == 0, "tile loads split evenly over the workgroup");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  GemmTiles<R>& t = sh.gt;
+  const int nt = (n + GT - 1) / GT;
+  for (int tj = 0; tj < nt; ++tj) {
+    for (int ti = LOWER_ONLY ? tj : 0; ti < nt; ++ti) {
+      const int i0 = ti * GT, j0 = tj * GT;
+      int klo = 0, khi = 0;
+      krange(i0, j0, klo, khi);
+      cplx<R> acc[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) acc[c] = mk<R>(0, 0);
+      cplx<R> ra[PER], rb[PER];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * NT_;
+          const int ka = A_KFAST ? e % GK : e / GT;
+          const int ia = i0 + (A_KFAST ? e / GK : e % GT);
+          ra[q] = (ia < n && k0 + ka < khi) ? fa(ia, k0 + ka) : mk<R>(0, 0);
+          const int kb = B_KFAST ? e % GK : e / GT;
+          const int jb = j0 + (B_KFAST ? e / GK : e % GT);
+          rb[q] = (jb < n && k0 + kb < khi) ? fb(k0 + kb, jb) : mk<R>(0, 0);
+        }
+      };
+      if (klo < khi) fetch(klo);
+      for (int k0 = klo; k0 < khi; k0 += GK) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * NT_;
+          t.a[A_KFAST ? e % GK : e / GT][A_KFAST ? e / GK : e % GT] = ra[q];
+          t.b[B_KFAST ? e % GK : e / GT][B_KFAST ? e / GK : e % GT] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + GK < khi) fetch(k0 + GK);
+#pragma unroll 4
+        for (int k = 0; k < GK; ++k) {
+          const cplx<R> a = t.a[k][lane];
+#pragma unroll
+          for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a * t.b[k][wave * CPW + c];
+        }
+        __syncthreads();
+      }
+      const int i = i0 + lane;
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        const int j = j0 + wave * CPW + c;
+        if (i < n && j < n) C[i + (size_t)j * ldc] = acc[c];
+      }
     }
-    for (int i = tid; i < k; i += NT_) sh.v[i] = conj(L[k + (size_t)i * ldl]);
-    __syncthreads();
-    cols_update<R, NMAX, false>(0, n, k, [](int) { return 0; }, B, ldb, sh.v, sh.w);
-    __syncthreads();
   }
 }
 
@@ -298,97 +396,189 @@ __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<
   __syncthreads();
 }
 
+// r = sqrt(x) and 1/r for x > 0 from the hardware reciprocal square root and two coupled
+// Newton steps (g -> sqrt(x), h -> 1/(2 sqrt(x)); each step squares the relative error),
+// so that the rotation chain holds no correctly rounded square root or division.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double& r, double& rinv) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double t = fma(-g, h, 0.5);
+  g = fma(g, t, g);
+  h = fma(h, t, h);
+  t = fma(-g, h, 0.5);
+  g = fma(g, t, g);
+  h = fma(h, t, h);
+  r = g;
+  rinv = 2.0 * h;
+}
+__device__ __forceinline__ void sqrt_rsqrt(float x, float& r, float& rinv) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  float g = x * y, h = 0.5f * y;
+  const float t = fmaf(-g, h, 0.5f);
+  r = fmaf(g, t, g);
+  rinv = 2.0f * fmaf(h, t, h);
+}
+
+// Smallest and largest x = pp^2 + e^2 for which sqrt_rsqrt is used (outside: exact path).
+template <typename R>
+__device__ __forceinline__ R rot_lo();
+template <>
+__device__ __forceinline__ double rot_lo<double>() { return 1e-290; }
+template <>
+__device__ __forceinline__ float rot_lo<float>() { return 1e-35f; }
+template <typename R>
+__device__ __forceinline__ R rot_hi();
+template <>
+__device__ __forceinline__ double rot_hi<double>() { return 1e300; }
+template <>
+__device__ __forceinline__ float rot_hi<float>() { return 1e36f; }
+
 // Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
 // next sweep and stores that sweep's rotations i = m-1 .. l in cs/sn[buf]; returns false
 // when every eigenvalue has converged (or the iteration limit was hit, ql.fail = 1).
-// Runs on one thread; d, e live in LDS.
+// Runs on all 64 lanes of one wavefront: the state is uniform, the deflation scan and the
+// shift of d are split over the lanes, and the rotation chain runs on lane 0 with the
+// next d[i], e[i] loaded one step ahead.  d, e and the state live in LDS.
 template <typename R, int NMAX>
-__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf) {
+__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
+                              unsigned long long& n_rot) {
   QlState<R>& q = sh.ql;
   R* d = sh.dd;
   R* e = sh.ee;
   const R eps = eps_r<R>();
-  while (q.l < n) {
-    const int l = q.l;
-    if (!q.in_iter) {
-      q.tst1 = fmax(q.tst1, fabs(d[l]) + fabs(e[l]));
-      int m = l;
-      while (m < n - 1 && fabs(e[m]) > eps * q.tst1) ++m;
+  int l = q.l, m = q.m, iter = q.iter, in_iter = q.in_iter;
+  R f = q.f, tst1 = q.tst1;
+  bool more = false;
+  while (l < n) {
+    if (!in_iter) {
+      tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
+      // m = first index >= l with |e[m]| <= eps tst1 (NaN stops the scan), else n - 1
+      int mm = n - 1;
+      for (int b = l; b < n - 1; b += 64) {
+        const int k = b + lane;
+        const bool small = k < n - 1 && !(fabs(e[k]) > eps * tst1);
+        const unsigned long long bal = __ballot(small);
+        if (bal) {
+          mm = b + __ffsll(static_cast<long long>(bal)) - 1;
+          break;
+        }
+      }
+      m = mm;
       if (m == l) {
-        d[l] += q.f;
-        e[l] = 0;
-        ++q.l;
+        if (lane == 0) {
+          d[l] += f;
+          e[l] = 0;
+        }
+        wave_sync();
+        ++l;
         continue;
       }
-      q.m = m;
-      q.in_iter = 1;
-      q.iter = 0;
+      in_iter = 1;
+      iter = 0;
     }
-    if (++q.iter > 30 * n) {
-      q.fail = 1;
-      q.l = n;
-      return false;
+    if (++iter > 30 * n) {
+      if (lane == 0) q.fail = 1;
+      l = n;
+      break;
     }
-    const int m = q.m;
-    R g = d[l];
-    R pp = (d[l + 1] - g) / (R(2) * e[l]);
+    // shift from the leading 2x2 block (uniform over the lanes)
+    const R g0 = d[l];
+    const R el = e[l];
+    R pp = (d[l + 1] - g0) / (R(2) * el);
     R r = sqrt(pp * pp + R(1));
     if (pp < 0) r = -r;
-    d[l] = e[l] / (pp + r);
-    d[l + 1] = e[l] * (pp + r);
-    const R dl1 = d[l + 1];
-    R h = g - d[l];
-    for (int i = l + 2; i < n; ++i) d[i] -= h;
-    q.f += h;
-    pp = d[m];
-    R c = 1, c2 = 1, c3 = 1, s = 0, s2 = 0;
-    const R el1 = e[l + 1];
-    R* cs = sh.cs[buf];
-    R* sn = sh.sn[buf];
-    for (int i = m - 1; i >= l; --i) {
-      c3 = c2;
-      c2 = c;
-      s2 = s;
-      g = c * e[i];
-      h = c * pp;
-      // Givens rotation zeroing e[i]: one square root and one reciprocal
-      const R ei = e[i];
-      r = sqrt(pp * pp + ei * ei);
-      e[i + 1] = s * r;
-      if (r != R(0)) {
-        const R rinv = R(1) / r;
-        s = ei * rinv;
-        c = pp * rinv;
-      } else {
-        s = R(0);
-        c = R(1);
+    const R dl = el / (pp + r);
+    const R dl1 = el * (pp + r);
+    const R h0 = g0 - dl;
+    for (int i = l + 2 + lane; i < n; i += 64) d[i] -= h0;
+    f += h0;
+    wave_sync();  // lane 0 reads the shifted d below
+    int conv = 0;
+    if (lane == 0) {
+      d[l] = dl;
+      d[l + 1] = dl1;
+      pp = d[m];
+      R c = 1, c2 = 1, c3 = 1, s = 0, s2 = 0;
+      const R el1 = e[l + 1];
+      R* cs = sh.cs[buf];
+      R* sn = sh.sn[buf];
+      R ei = e[m - 1], di = d[m - 1];
+      for (int i = m - 1; i >= l; --i) {
+        // operands of the next step (not written by this sweep before they are used),
+        // loaded unconditionally from an in-range index so that no branch separates the
+        // loads from their use one step later
+        const int ip = i > l ? i - 1 : i;
+        const R ein = e[ip];
+        const R din = d[ip];
+        c3 = c2;
+        c2 = c;
+        s2 = s;
+        const R g = c * ei;
+        const R h = c * pp;
+        // Givens rotation zeroing e[i]
+        const R x = pp * pp + ei * ei;
+        R rr, rinv;
+        if (x >= rot_lo<R>() && x <= rot_hi<R>()) {
+          sqrt_rsqrt(x, rr, rinv);
+          e[i + 1] = s * rr;
+          s = ei * rinv;
+          c = pp * rinv;
+        } else {
+          rr = sqrt(x);
+          e[i + 1] = s * rr;
+          if (rr != R(0) && rr == rr) {
+            s = ei / rr;
+            c = pp / rr;
+          } else {
+            s = R(0);
+            c = R(1);
+          }
+        }
+        pp = c * di - s * g;
+        d[i + 1] = h + s * (c * g + s * di);
+        cs[i] = c;
+        sn[i] = s;
+        ei = ein;
+        di = din;
       }
-      pp = c * d[i] - s * g;
-      d[i + 1] = h + s * (c * g + s * d[i]);
-      cs[i] = c;
-      sn[i] = s;
+      pp = -s * s2 * c3 * el1 * el / dl1;
+      e[l] = s * pp;
+      d[l] = c * pp;
+      sh.sweep[buf][0] = l;
+      sh.sweep[buf][1] = m;
+      if (!(fabs(e[l]) > eps * tst1)) {
+        // converged: this sweep is the last one for l
+        d[l] += f;
+        e[l] = 0;
+        conv = 1;
+      }
+      n_rot += static_cast<unsigned long long>(m - l);
     }
-    pp = -s * s2 * c3 * el1 * e[l] / dl1;
-    e[l] = s * pp;
-    d[l] = c * pp;
-    sh.sweep[buf][0] = l;
-    sh.sweep[buf][1] = m;
-    if (!(fabs(e[l]) > eps * q.tst1)) {
-      // converged: this sweep is the last one for l
-      d[l] += q.f;
-      e[l] = 0;
-      ++q.l;
-      q.in_iter = 0;
+    conv = __shfl(conv, 0);
+    wave_sync();
+    if (conv) {
+      ++l;
+      in_iter = 0;
     }
-    return true;
+    more = true;
+    break;
   }
-  return false;
+  if (lane == 0) {
+    q.l = l;
+    q.m = m;
+    q.iter = iter;
+    q.in_iter = in_iter;
+    q.f = f;
+    q.tst1 = tst1;
+  }
+  wave_sync();
+  return more;
 }
 
 // records the wall clock at phase boundaries when the caller asked for a profile
 template <typename R>
 __device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
-  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 8 + k] = wall_clock64();
+  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 16 + k] = wall_clock64();
 }
 
 template <typename R, typename TIn, int NMAX>
@@ -480,17 +670,36 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
       __syncthreads();
     }
   eig_mark(p, sid, 2);
-    // ---- C = L^-1 S L^-H ---------------------------------------------------------------
-    fwd_solve<R, NMAX>(n, L, ld, A, ld, sh);  // A = L^-1 S
-    for (int j = wave; j < n; j += NW_)
-      for (int i = lane; i < n; i += 64) X[i + (size_t)j * ldx] = conj(A[j + (size_t)i * ld]);
+    // ---- C = W S W^H with W = L^-1 -----------------------------------------------------
+    tri_inverse<R, NMAX>(n, L, ld, sh);  // lower triangle of L holds W
+    const cplx<R>* W = L;
+    // X = W S (W lower triangular: k <= i)
+    gemm_tiles<R, NMAX, false, true, false>(
+        n, [=](int i, int k) { return k <= i ? W[i + (size_t)k * ld] : mk<R>(0, 0); },
+        [=](int k, int j) { return A[k + (size_t)j * ld]; },
+        [=](int i0, int, int& klo, int& khi) {
+          klo = 0;
+          khi = min(n, i0 + GT);
+        },
+        X, ldx, sh);
     __syncthreads();
-    fwd_solve<R, NMAX>(n, L, ld, X, ldx, sh);  // X = L^-1 (L^-1 S)^H = C
+    // A = X W^H on and below the diagonal ((W^H)[k, j] = conj(W[j, k]), k <= j), then the
+    // strict upper triangle mirrored from the lower one and a real diagonal
+    gemm_tiles<R, NMAX, false, false, true>(
+        n, [=](int i, int k) { return X[i + (size_t)k * ldx]; },
+        [=](int k, int j) { return k <= j ? conj(W[j + (size_t)k * ld]) : mk<R>(0, 0); },
+        [=](int, int j0, int& klo, int& khi) {
+          klo = 0;
+          khi = min(n, j0 + GT);
+        },
+        A, ld, sh);
+    __syncthreads();
     for (int j = wave; j < n; j += NW_)
-      for (int i = lane; i < n; i += 64) {
-        cplx<R> c = (i >= j) ? X[i + (size_t)j * ldx] : conj(X[j + (size_t)i * ldx]);
-        if (i == j) c.im = 0;
-        A[i + (size_t)j * ld] = c;
+      for (int i = lane; i <= j && i < n; i += 64) {
+        if (i < j)
+          A[i + (size_t)j * ld] = conj(A[j + (size_t)i * ld]);
+        else
+          A[i + (size_t)i * ld].im = 0;
       }
     __syncthreads();
   }
@@ -544,56 +753,85 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 4);
 
   // ---- implicit QL on the real tridiagonal (tql2), rotations into Z -------------------
-  // A generator thread produces the rotations of sweep t+1 while the row owners apply
-  // those of sweep t (double-buffered); the generator is a thread without a row of Z when
+  // A generator wavefront produces the rotations of sweep t+1 while the row owners apply
+  // those of sweep t (double-buffered); the generator is a wavefront without rows of Z when
   // the workgroup has one.
   for (int j = wave; j < n; j += NW_)
     for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
-  const int gen = n <= NT_ - 64 ? NT_ - 64 : 0;
-  if (tid == gen) {
-    sh.ql.f = 0;
-    sh.ql.tst1 = 0;
-    sh.ql.l = 0;
-    sh.ql.m = 0;
-    sh.ql.iter = 0;
-    sh.ql.in_iter = 0;
-    sh.ql.fail = 0;
-    sh.sweep[0][2] = ql_next_sweep<R, NMAX>(n, sh, 0) ? 1 : 0;
+  const int gwave = n <= NT_ - 64 ? NW_ - 1 : 0;
+  unsigned long long gen_cycles = 0, n_rot = 0, n_sweep = 0;
+  if (wave == gwave) {
+    if (lane == 0) {
+      sh.ql.f = 0;
+      sh.ql.tst1 = 0;
+      sh.ql.l = 0;
+      sh.ql.m = 0;
+      sh.ql.iter = 0;
+      sh.ql.in_iter = 0;
+      sh.ql.fail = 0;
+    }
+    wave_sync();
+    const unsigned long long t0 = p.prof ? wall_clock64() : 0;
+    const bool more = ql_next_sweep<R, NMAX>(n, sh, 0, lane, n_rot);
+    if (p.prof) gen_cycles += wall_clock64() - t0;
+    if (lane == 0) sh.sweep[0][2] = more ? 1 : 0;
   }
   __syncthreads();
   for (int t = 0;; ++t) {
     const int cur = t & 1;
     if (!sh.sweep[cur][2]) break;
     const int l = sh.sweep[cur][0], m = sh.sweep[cur][1];
-    if (tid == gen) sh.sweep[cur ^ 1][2] = ql_next_sweep<R, NMAX>(n, sh, cur ^ 1) ? 1 : 0;
+    ++n_sweep;
+    if (wave == gwave) {
+      const unsigned long long t0 = p.prof ? wall_clock64() : 0;
+      const bool more = ql_next_sweep<R, NMAX>(n, sh, cur ^ 1, lane, n_rot);
+      if (p.prof) gen_cycles += wall_clock64() - t0;
+      if (lane == 0) sh.sweep[cur ^ 1][2] = more ? 1 : 0;
+    }
     // a sweep always satisfies 0 <= l < m < n; anything else is skipped, never indexed
     const bool sweep_ok = l >= 0 && l < m && m < n;
     // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
-    // fetched CH at a time so that their loads are in flight together
-    constexpr int CH = 32;
+    // fetched CH at a time, the loads of the next chunk issued before the current chunk
+    // is rotated and stored (the chunks cover disjoint columns)
+    constexpr int CH = 24;
     const R* cs = sh.cs[cur];
     const R* sn = sh.sn[cur];
     for (int row = sweep_ok ? tid : n; row < n; row += NT_) {
-      R carry = Z[row + (size_t)m * ld];
-      for (int i0 = m - 1; i0 >= l; i0 -= CH) {
-        const int cnt = min(CH, i0 - l + 1);
-        R zc[CH];
+      R* zr = Z + row;
+      R carry = zr[(size_t)m * ld];
+      R za[CH], zb[CH];
+      auto fetch = [&](R* buf, int i0) {
 #pragma unroll
         for (int u = 0; u < CH; ++u)
-          if (u < cnt) zc[u] = Z[row + (size_t)(i0 - u) * ld];
+          if (i0 - u >= l) buf[u] = zr[(size_t)(i0 - u) * ld];
+      };
+      auto rotate = [&](const R* buf, int i0) {
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          if (u < cnt) {
-            const int i = i0 - u;
+          const int i = i0 - u;
+          if (i >= l) {
             const R c = cs[i], s = sn[i];
-            Z[row + (size_t)(i + 1) * ld] = s * zc[u] + c * carry;
-            carry = c * zc[u] - s * carry;
+            zr[(size_t)(i + 1) * ld] = s * buf[u] + c * carry;
+            carry = c * buf[u] - s * carry;
           }
         }
+      };
+      fetch(za, m - 1);
+      for (int i0 = m - 1; i0 >= l; i0 -= 2 * CH) {
+        if (i0 - CH >= l) fetch(zb, i0 - CH);
+        rotate(za, i0);
+        if (i0 - CH < l) break;
+        if (i0 - 2 * CH >= l) fetch(za, i0 - 2 * CH);
+        rotate(zb, i0 - CH);
       }
-      Z[row + (size_t)l * ld] = carry;
+      zr[(size_t)l * ld] = carry;
     }
     __syncthreads();
+  }
+  if (p.prof && wave == gwave && lane == 0) {
+    p.prof[(size_t)sid * 16 + 8] = gen_cycles;
+    p.prof[(size_t)sid * 16 + 9] = n_sweep;
+    p.prof[(size_t)sid * 16 + 10] = n_rot;
   }
   if (sh.ql.fail) {
     for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
@@ -676,8 +914,24 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   }
 
   eig_mark(p, sid, 6);
-  // ---- un-whiten: eigenvectors of the pencil = L^-H X ---------------------------------
-  if (!p.standard) bwd_solve_h<R, NMAX>(n, L, ld, X, ldx, sh);
+  // ---- un-whiten: eigenvectors of the pencil = W^H X ----------------------------------
+  if (!p.standard) {
+    // X is copied into A (free after the back-transform), then X = W^H A with
+    // (W^H)[i, k] = conj(W[k, i]) for k >= i
+    for (int j = wave; j < n; j += NW_)
+      for (int i = lane; i < n; i += 64) A[i + (size_t)j * ld] = X[i + (size_t)j * ldx];
+    __syncthreads();
+    const cplx<R>* W = L;
+    gemm_tiles<R, NMAX, true, true, false>(
+        n, [=](int i, int k) { return k >= i ? conj(W[k + (size_t)i * ld]) : mk<R>(0, 0); },
+        [=](int k, int j) { return A[k + (size_t)j * ld]; },
+        [=](int i0, int, int& klo, int& khi) {
+          klo = i0;
+          khi = n;
+        },
+        X, ldx, sh);
+    __syncthreads();
+  }
   eig_mark(p, sid, 7);
   if (tid == 0) p.info[sid] = 0;
   __syncthreads();

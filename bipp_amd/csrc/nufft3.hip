@@ -36,14 +36,6 @@ constexpr int SP_WMAX = 16;  // largest kernel width
 constexpr int SP_MAXR = 9;   // neighbour bin ranges per block: (nback + 1)^2 with nback <= 2
 constexpr int FFT_NT = 512;  // FFT workgroup: ~2 radix-4 butterflies per thread per stage
 
-// Orders the LDS accesses of the lanes of one wavefront around a hand-off through LDS
-// (lanes write, other lanes of the same wavefront read afterwards, no workgroup barrier).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <typename R>
 __device__ __forceinline__ R es_kernel(R z, R beta) {
   const R t = R(1) - z * z;
