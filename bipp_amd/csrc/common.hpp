@@ -111,6 +111,32 @@ __device__ __forceinline__ R block_sum(R v, R* scratch) {
   return s;
 }
 
+// Lane moves within rows of 16 lanes (DPP, no LDS traffic).  CTRL: 0xB1 quad_perm [1,0,3,2],
+// 0x4E quad_perm [2,3,0,1], 0x141 row_half_mirror, 0x140 row_mirror.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// Sum over each aligned group of 16 lanes, returned in every lane of the group; every lane
+// adds the same operands (in commuted order), so all lanes hold bit-identical sums.
+template <typename R>
+__device__ __forceinline__ R sum16(R v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return v;
+}
+
 // Orders the LDS accesses of the lanes of one wavefront around a hand-off through LDS
 // (lanes write, other lanes of the same wavefront read afterwards, no workgroup barrier).
 __device__ __forceinline__ void wave_sync() {

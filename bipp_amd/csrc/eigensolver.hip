@@ -80,6 +80,22 @@ struct GemmTiles {
   cplx<R> b[GK][GT + 1];  // b[k][c] = op(B)[k0 + k, j0 + c]
 };
 
+// Back-transform layout: LPC lanes hold one eigenvector column (RR rows each), reflectors
+// staged through LDS RB at a time.
+template <int NMAX>
+struct BtCfg {
+  static constexpr int LPC = NMAX >= 512 ? 32 : 16;
+  static constexpr int RR = NMAX / LPC;
+  static constexpr int CPWV = 64 / LPC;
+  static constexpr int RB = NMAX >= 512 ? 4 : 8;
+};
+
+template <typename R, int NMAX>
+struct BackTile {
+  cplx<R> v[BtCfg<NMAX>::RB][NMAX];  // reflector vectors (zero above the unit entry)
+  cplx<R> tau[BtCfg<NMAX>::RB];
+};
+
 template <typename R, int NMAX>
 struct EigShared {
   static constexpr int NW_ = EigCfg<NMAX>::NW;
@@ -89,6 +105,7 @@ struct EigShared {
   union {
     cplx<R> xw[NW_][NMAX];  // per-wave partial sums of the matrix-vector passes
     GemmTiles<R> gt;        // operand tiles of the GEMM passes
+    BackTile<R, NMAX> bt;   // reflectors of the back-transform
   };
   R dd[NMAX];
   R ee[NMAX];
@@ -858,55 +875,100 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.dd[sh.perm[j]] : R(0);
 
   // ---- back-transform X = Q Z[:, perm] ------------------------------------------------
-  for (int j = wave; j < n; j += NW_) {
-    const int pj = sh.perm[j];
-    for (int i = lane; i < n; i += 64) X[i + (size_t)j * ldx] = mk<R>(Z[i + (size_t)pj * ld], 0);
-  }
-  __syncthreads();
-  for (int c = n - 2; c >= 0; --c) {
-    const cplx<R> tau = A[c + (size_t)c * ld];
-    if (tau.re == R(0) && tau.im == R(0)) continue;  // uniform: every thread reads the same value
-    for (int i = c + 1 + tid; i < n; i += NT_)
-      sh.v[i] = (i == c + 1) ? mk<R>(1, 0) : A[i + (size_t)c * ld];
-    __syncthreads();
-    // X[c+1:, j] -= tau v (v^H X[c+1:, j]); UC columns per wave loaded together
-    for (int jb = wave; jb < n; jb += NW_ * UC) {
-      cplx<R> x[UC][RM];
+  // Each group of LPC lanes keeps one column of Z[:, perm] in registers (rows g + LPC r)
+  // while H_{n-2}, ..., H_0 are applied to it, the reflectors staged through LDS RB at a
+  // time; row blocks that lie above a reflector's unit entry are skipped.
+  {
+    constexpr int LPC = BtCfg<NMAX>::LPC, RR = BtCfg<NMAX>::RR, CPWV = BtCfg<NMAX>::CPWV,
+                  BT_RB = BtCfg<NMAX>::RB, RG = RR >= 4 ? 4 : RR;
+    static_assert(LPC == 16 || LPC == 32, "column groups of 16 or 32 lanes");
+    BackTile<R, NMAX>& bt = sh.bt;
+    const int g = lane % LPC;
+    for (int jb = 0; jb < n; jb += NW_ * CPWV) {
+      const int j = jb + wave * CPWV + lane / LPC;
+      const bool live = j < n;
+      const int pj = live ? sh.perm[j] : 0;
+      cplx<R> x[RR];
 #pragma unroll
-      for (int u = 0; u < UC; ++u) {
-        const int j = jb + u * NW_;
-#pragma unroll
-        for (int r = 0; r < RM; ++r) {
-          const int i = c + 1 + lane + 64 * r;
-          if (j < n && i < n) x[u][r] = X[i + (size_t)j * ldx];
-        }
+      for (int r = 0; r < RR; ++r) {
+        const int i = g + LPC * r;
+        x[r] = mk<R>(live && i < n ? Z[i + (size_t)pj * ld] : R(0), R(0));
       }
-      cplx<R> t[UC];
+      // the chunk of reflectors cb, cb - 1, ... is fetched into registers one chunk ahead
+      constexpr int PER = BT_RB * NMAX / NT_;
+      static_assert(BT_RB * NMAX % NT_ == 0, "reflector chunk splits evenly");
+      cplx<R> rv[PER], rt = mk<R>(0, 0);
+      auto fetch = [&](int cb) {
 #pragma unroll
-      for (int u = 0; u < UC; ++u) {
-        const int j = jb + u * NW_;
-        R pr = 0, pi = 0;
+        for (int u = 0; u < PER; ++u) {
+          const int e = tid + u * NT_;
+          const int q = e / NMAX, i = e % NMAX, c = cb - q;
+          cplx<R> val = mk<R>(0, 0);
+          if (c >= 0 && i < n) {
+            if (i == c + 1)
+              val = mk<R>(1, 0);
+            else if (i > c + 1)
+              val = A[i + (size_t)c * ld];
+          }
+          rv[u] = val;
+        }
+        if (tid < BT_RB)
+          rt = cb - tid >= 0 ? A[(cb - tid) + (size_t)(cb - tid) * ld] : mk<R>(0, 0);
+      };
+      if (n >= 2) fetch(n - 2);
+      for (int cb = n - 2; cb >= 0; cb -= BT_RB) {
+        __syncthreads();  // the previous chunk is no longer read
 #pragma unroll
-        for (int r = 0; r < RM; ++r) {
-          const int i = c + 1 + lane + 64 * r;
-          if (j < n && i < n) {
-            const cplx<R> q = cmul_conj_a(sh.v[i], x[u][r]);
-            pr += q.re;
-            pi += q.im;
+        for (int u = 0; u < PER; ++u) {
+          const int e = tid + u * NT_;
+          bt.v[e / NMAX][e % NMAX] = rv[u];
+        }
+        if (tid < BT_RB) bt.tau[tid] = rt;
+        __syncthreads();
+        if (cb - BT_RB >= 0) fetch(cb - BT_RB);
+        for (int q = 0; q < BT_RB; ++q) {
+          const int c = cb - q;
+          if (c < 0) break;
+          const cplx<R> tau = bt.tau[q];
+          if (tau.re == R(0) && tau.im == R(0)) continue;  // uniform over the workgroup
+          // row blocks r < RG g0 hold rows <= c only, where v = 0: skipped RG blocks at a
+          // time (a uniform branch per group, the group's LDS loads issued together)
+          const int g0 = (c + 1) / (LPC * RG);
+          cplx<R> dt = mk<R>(0, 0);
+#pragma unroll
+          for (int gq = 0; gq < RR / RG; ++gq) {
+            if (gq >= g0) {
+              cplx<R> vv[RG];
+#pragma unroll
+              for (int u = 0; u < RG; ++u) vv[u] = bt.v[q][g + LPC * (gq * RG + u)];
+#pragma unroll
+              for (int u = 0; u < RG; ++u) dt = dt + cmul_conj_a(vv[u], x[gq * RG + u]);
+            }
+          }
+          dt.re = sum16(dt.re);
+          dt.im = sum16(dt.im);
+          if (LPC == 32) {
+            dt.re += __shfl_xor(dt.re, 16, 64);
+            dt.im += __shfl_xor(dt.im, 16, 64);
+          }
+          const cplx<R> t = tau * dt;
+#pragma unroll
+          for (int gq = 0; gq < RR / RG; ++gq) {
+            if (gq >= g0) {
+              cplx<R> vv[RG];
+#pragma unroll
+              for (int u = 0; u < RG; ++u) vv[u] = bt.v[q][g + LPC * (gq * RG + u)];
+#pragma unroll
+              for (int u = 0; u < RG; ++u) x[gq * RG + u] = x[gq * RG + u] - vv[u] * t;
+            }
           }
         }
-        pr = wave_sum(pr);
-        pi = wave_sum(pi);
-        t[u] = tau * mk<R>(pr, pi);
       }
+      if (live) {
 #pragma unroll
-      for (int u = 0; u < UC; ++u) {
-        const int j = jb + u * NW_;
-        if (j >= n) continue;
-#pragma unroll
-        for (int r = 0; r < RM; ++r) {
-          const int i = c + 1 + lane + 64 * r;
-          if (i < n) X[i + (size_t)j * ldx] = x[u][r] - sh.v[i] * t[u];
+        for (int r = 0; r < RR; ++r) {
+          const int i = g + LPC * r;
+          if (i < n) X[i + (size_t)j * ldx] = x[r];
         }
       }
     }
