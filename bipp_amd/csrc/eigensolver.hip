@@ -26,8 +26,9 @@ constexpr int NT = 256;
 constexpr int NW = NT / 64;
 
 // Eigensolver launch shape per size bucket: wider workgroups for the larger problems, so
-// that more columns are in flight per step; the per-sample matrices live in global memory
-// (HBM at these sizes), so every step is bound by the number of dependent round trips.
+// that more columns are in flight per step; the per-sample matrices live in global memory,
+// so most steps are bound by dependent round trips.  (Two 256-thread workgroups per CU for
+// NMAX = 256, which overlaps one sample's QL chain with the other's work, measured slower.)
 template <int NMAX>
 struct EigCfg {
   static constexpr int NT = NMAX >= 256 ? 512 : 256;
@@ -413,42 +414,36 @@ __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<
   __syncthreads();
 }
 
-// r = sqrt(x) and 1/r for x > 0 from the hardware reciprocal square root and two coupled
-// Newton steps (g -> sqrt(x), h -> 1/(2 sqrt(x)); each step squares the relative error),
-// so that the rotation chain holds no correctly rounded square root or division.
-__device__ __forceinline__ void sqrt_rsqrt(double x, double& r, double& rinv) {
+// g = sqrt(x) and h = 1/(2 sqrt(x)) for normal x > 0 from the hardware reciprocal square
+// root (about 2^-25 relative) and coupled Newton steps (each squares the relative error;
+// two steps reach 1 ulp in double), so that the rotation chain holds no correctly rounded
+// square root or division.
+__device__ __forceinline__ void sqrt_half_rsqrt(double x, double& g, double& h) {
   const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y, h = 0.5 * y;
+  g = x * y;
+  h = 0.5 * y;
   double t = fma(-g, h, 0.5);
   g = fma(g, t, g);
   h = fma(h, t, h);
   t = fma(-g, h, 0.5);
   g = fma(g, t, g);
   h = fma(h, t, h);
-  r = g;
-  rinv = 2.0 * h;
 }
-__device__ __forceinline__ void sqrt_rsqrt(float x, float& r, float& rinv) {
+__device__ __forceinline__ void sqrt_half_rsqrt(float x, float& g, float& h) {
   const float y = __builtin_amdgcn_rsqf(x);
-  float g = x * y, h = 0.5f * y;
+  g = x * y;
+  h = 0.5f * y;
   const float t = fmaf(-g, h, 0.5f);
-  r = fmaf(g, t, g);
-  rinv = 2.0f * fmaf(h, t, h);
+  g = fmaf(g, t, g);
+  h = fmaf(h, t, h);
 }
 
-// Smallest and largest x = pp^2 + e^2 for which sqrt_rsqrt is used (outside: exact path).
 template <typename R>
-__device__ __forceinline__ R rot_lo();
+__device__ __forceinline__ R min_normal();
 template <>
-__device__ __forceinline__ double rot_lo<double>() { return 1e-290; }
+__device__ __forceinline__ double min_normal<double>() { return 2.2250738585072014e-308; }
 template <>
-__device__ __forceinline__ float rot_lo<float>() { return 1e-35f; }
-template <typename R>
-__device__ __forceinline__ R rot_hi();
-template <>
-__device__ __forceinline__ double rot_hi<double>() { return 1e300; }
-template <>
-__device__ __forceinline__ float rot_hi<float>() { return 1e36f; }
+__device__ __forceinline__ float min_normal<float>() { return 1.17549435e-38f; }
 
 // Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
 // next sweep and stores that sweep's rotations i = m-1 .. l in cs/sn[buf]; returns false
@@ -532,26 +527,20 @@ __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
         s2 = s;
         const R g = c * ei;
         const R h = c * pp;
-        // Givens rotation zeroing e[i]
-        const R x = pp * pp + ei * ei;
-        R rr, rinv;
-        if (x >= rot_lo<R>() && x <= rot_hi<R>()) {
-          sqrt_rsqrt(x, rr, rinv);
-          e[i + 1] = s * rr;
-          s = ei * rinv;
-          c = pp * rinv;
-        } else {
-          rr = sqrt(x);
-          e[i + 1] = s * rr;
-          if (rr != R(0) && rr == rr) {
-            s = ei / rr;
-            c = pp / rr;
-          } else {
-            s = R(0);
-            c = R(1);
-          }
-        }
-        pp = c * di - s * g;
+        // Givens rotation zeroing e[i]: r = sqrt(pp^2 + e_i^2), c = pp / r, s = e_i / r, and
+        // the next pp = c d_i - s g = (pp d_i - e_i g) / r, whose numerator is formed while
+        // the square root is refined (the dependency chain runs through r only); r = 0
+        // (pp = e_i = 0, or below the normal range) is the identity rotation
+        const R x = fma(pp, pp, ei * ei);
+        const R wnum = fma(pp, di, -(ei * g));
+        R rr, hr;
+        sqrt_half_rsqrt(x, rr, hr);
+        const bool ok = x >= min_normal<R>();
+        const R rinv = hr + hr;
+        e[i + 1] = ok ? s * rr : R(0);
+        s = ok ? ei * rinv : R(0);
+        c = ok ? pp * rinv : R(1);
+        pp = ok ? wnum * rinv : di;
         d[i + 1] = h + s * (c * g + s * di);
         cs[i] = c;
         sn[i] = s;
@@ -770,12 +759,13 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 4);
 
   // ---- implicit QL on the real tridiagonal (tql2), rotations into Z -------------------
-  // A generator wavefront produces the rotations of sweep t+1 while the row owners apply
-  // those of sweep t (double-buffered); the generator is a wavefront without rows of Z when
-  // the workgroup has one.
+  // The last wavefront produces the rotations of sweep t+1 while the other threads apply
+  // those of sweep t to their rows of Z (double-buffered).
   for (int j = wave; j < n; j += NW_)
     for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
-  const int gwave = n <= NT_ - 64 ? NW_ - 1 : 0;
+  static_assert(NW_ >= 2, "a generator wavefront besides the row owners");
+  const int gwave = NW_ - 1;
+  constexpr int NRT = NT_ - 64;  // row-owner threads
   unsigned long long gen_cycles = 0, n_rot = 0, n_sweep = 0;
   if (wave == gwave) {
     if (lane == 0) {
@@ -813,7 +803,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     constexpr int CH = 24;
     const R* cs = sh.cs[cur];
     const R* sn = sh.sn[cur];
-    for (int row = sweep_ok ? tid : n; row < n; row += NT_) {
+    for (int row = (sweep_ok && wave != gwave) ? tid : n; row < n; row += NRT) {
       R* zr = Z + row;
       R carry = zr[(size_t)m * ld];
       R za[CH], zb[CH];
