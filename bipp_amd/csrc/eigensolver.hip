@@ -97,21 +97,26 @@ struct BackTile {
   cplx<R> tau[BtCfg<NMAX>::RB];
 };
 
+// Two reals accessed together (one 2 x sizeof(R) LDS access).
+template <typename R>
+struct alignas(2 * sizeof(R)) RPair {
+  R x, y;
+};
+
 template <typename R, int NMAX>
 struct EigShared {
   static constexpr int NW_ = EigCfg<NMAX>::NW;
   cplx<R> v[NMAX];
   cplx<R> w[NMAX];
   cplx<R> vn[NMAX];
+  cplx<R> xc[NMAX];  // column (conjugate-transpose) parts of the Hermitian products
   union {
     cplx<R> xw[NW_][NMAX];  // per-wave partial sums of the matrix-vector passes
     GemmTiles<R> gt;        // operand tiles of the GEMM passes
     BackTile<R, NMAX> bt;   // reflectors of the back-transform
   };
-  R dd[NMAX];
-  R ee[NMAX];
-  R cs[2][NMAX];
-  R sn[2][NMAX];
+  RPair<R> de[NMAX];      // tridiagonal: (diagonal d_i, subdiagonal e_i)
+  RPair<R> rot[2][NMAX];  // rotations (c_i, s_i) of two QL sweeps
   int perm[NMAX];
   int idx[NMAX];
   R red[2 * NW_];
@@ -318,14 +323,27 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
   if (tid == 0) {
     vec[c + 1] = mk<R>(1, 0);
     col[c + 1] = mk<R>(beta, 0);
-    sh.ee[c] = beta;
+    sh.de[c].y = beta;
   }
   __syncthreads();
   return tau;
 }
 
-// One pass over the trailing square [i0,n)^2 of the full Hermitian A:
-//   optionally A -= v w^H + w v^H (pending rank-2 update), then x = A * vn.
+// Sum over the 64 lanes of a wavefront (every lane gets it; DPP within rows of 16, then
+// two lane swaps).
+template <typename R>
+__device__ __forceinline__ R wave_total(R v) {
+  v = sum16(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// One pass over the lower triangle of the trailing square [i0,n)^2 of the Hermitian A
+// (the strict upper triangle is neither read nor written):
+//   optionally A -= v w^H + w v^H (pending rank-2 update), then x = A * vn, where each
+//   stored A[i, j] (i >= j) adds A[i, j] vn_j to x_i and, below the diagonal,
+//   conj(A[i, j]) vn_i to x_j (a per-column reduction over the wavefront's lanes).
 // Result x[i0:n) is written to sh.w (v and w are no longer needed afterwards).
 template <typename R, int NMAX, bool APPLY>
 __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const cplx<R>* v,
@@ -333,13 +351,14 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
                 UC = EigCfg<NMAX>::UC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  cplx<R> acc[RM], vi[RM], wi[RM];
+  cplx<R> acc[RM], vi[RM], wi[RM], vni[RM];
 #pragma unroll
   for (int r = 0; r < RM; ++r) {
     acc[r] = mk<R>(0, 0);
     const int i = i0 + lane + 64 * r;
     vi[r] = (APPLY && i < n) ? v[i] : mk<R>(0, 0);
     wi[r] = (APPLY && i < n) ? sh.w[i] : mk<R>(0, 0);
+    vni[r] = i < n ? vn[i] : mk<R>(0, 0);
   }
   for (int jb = i0 + wave; jb < n; jb += NW_ * UC) {
     cplx<R> a[UC][RM];
@@ -350,12 +369,15 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
 #pragma unroll
       for (int r = 0; r < RM; ++r) {
         const int i = i0 + lane + 64 * r;
-        if (j < n && i < n) a[u][r] = col[i];
+        if (j < n && i < n && i >= j) a[u][r] = col[i];
       }
     }
+    R pre[UC], pim[UC];
 #pragma unroll
     for (int u = 0; u < UC; ++u) {
       const int j = jb + u * NW_;
+      pre[u] = 0;
+      pim[u] = 0;
       if (j >= n) continue;
       cplx<R> cvj = mk<R>(0, 0), cwj = mk<R>(0, 0);
       if (APPLY) {
@@ -367,14 +389,31 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
 #pragma unroll
       for (int r = 0; r < RM; ++r) {
         const int i = i0 + lane + 64 * r;
-        if (i < n) {
+        if (i < n && i >= j) {
           cplx<R> x = a[u][r];
           if (APPLY) {
             x = x - (vi[r] * cwj + wi[r] * cvj);
             col[i] = x;
           }
           acc[r] = acc[r] + x * vnj;
+          if (i > j) {
+            const cplx<R> t = cmul_conj_a(x, vni[r]);
+            pre[u] += t.re;
+            pim[u] += t.im;
+          }
         }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {
+      pre[u] = wave_total(pre[u]);
+      pim[u] = wave_total(pim[u]);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        const int j = jb + u * NW_;
+        if (j < n) sh.xc[j] = mk<R>(pre[u], pim[u]);
       }
     }
   }
@@ -385,9 +424,9 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
   }
   __syncthreads();
   for (int i = i0 + tid; i < n; i += NT_) {
-    cplx<R> s = sh.xw[0][i];
+    cplx<R> s = sh.xc[i];
 #pragma unroll
-    for (int w = 1; w < NW_; ++w) s = s + sh.xw[w][i];
+    for (int w = 0; w < NW_; ++w) s = s + sh.xw[w][i];
     sh.w[i] = s;
   }
   __syncthreads();
@@ -445,30 +484,99 @@ __device__ __forceinline__ double min_normal<double>() { return 2.22507385850720
 template <>
 __device__ __forceinline__ float min_normal<float>() { return 1.17549435e-38f; }
 
+// Bounds of max |d_i| + |e_i| and min |e_i| over a sweep within which the rotation chain
+// needs no guard: pp^2 + e_i^2 stays normal and finite.
+template <typename R>
+__device__ __forceinline__ R chain_lo();
+template <>
+__device__ __forceinline__ double chain_lo<double>() { return 1e-150; }
+template <>
+__device__ __forceinline__ float chain_lo<float>() { return 1e-18f; }
+template <typename R>
+__device__ __forceinline__ R chain_hi();
+template <>
+__device__ __forceinline__ double chain_hi<double>() { return 1e150; }
+template <>
+__device__ __forceinline__ float chain_hi<float>() { return 1e17f; }
+
+// One step of a QL sweep at index i: the Givens rotation zeroing e_i, r = sqrt(pp^2 + e_i^2),
+// c = pp / r, s = e_i / r, and the next pp = c d_i - s g = (pp d_i - e_i g) / r, whose
+// numerator is formed while the square root is refined (the dependency chain runs through
+// r only).  Writes (d_{i+1}, e_{i+1}) and (c, s) with one LDS access each.  GUARD: r = 0
+// (pp = e_i = 0, or pp^2 + e_i^2 below the normal range) is the identity rotation.
+template <typename R, bool GUARD>
+__device__ __forceinline__ void ql_step(int i, R ei, R di, R& pp, R& c, R& s, RPair<R>* de,
+                                        RPair<R>* rot) {
+  const R g = c * ei;
+  const R h = c * pp;
+  const R x = fma(pp, pp, ei * ei);
+  const R wnum = fma(pp, di, -(ei * g));
+  R rr, hr;
+  sqrt_half_rsqrt(x, rr, hr);
+  const R rinv = hr + hr;
+  R enew;
+  if (GUARD) {
+    const bool ok = x >= min_normal<R>();
+    enew = ok ? s * rr : R(0);
+    s = ok ? ei * rinv : R(0);
+    c = ok ? pp * rinv : R(1);
+    pp = ok ? wnum * rinv : di;
+  } else {
+    enew = s * rr;
+    s = ei * rinv;
+    c = pp * rinv;
+    pp = wnum * rinv;
+  }
+  RPair<R> o;
+  o.x = h + s * (c * g + s * di);
+  o.y = enew;
+  de[i + 1] = o;
+  RPair<R> cs;
+  cs.x = c;
+  cs.y = s;
+  rot[i] = cs;
+}
+
+// Rotations i = m-1 .. l of one sweep on one lane, two steps per iteration with the operands
+// of the next two steps loaded ahead (they are not written by this sweep before use).
+template <typename R, bool GUARD>
+__device__ __forceinline__ void ql_chain(int l, int m, R& pp, R& c, R& s, RPair<R>* de,
+                                         RPair<R>* rot) {
+  RPair<R> cur = de[m - 1];
+  int i = m - 1;
+  for (; i - 1 >= l; i -= 2) {
+    const RPair<R> p1 = de[i - 1];
+    const RPair<R> p2 = de[i - 2 >= l ? i - 2 : i - 1];
+    ql_step<R, GUARD>(i, cur.y, cur.x, pp, c, s, de, rot);
+    ql_step<R, GUARD>(i - 1, p1.y, p1.x, pp, c, s, de, rot);
+    cur = p2;
+  }
+  if (i >= l) ql_step<R, GUARD>(i, cur.y, cur.x, pp, c, s, de, rot);
+}
+
 // Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
-// next sweep and stores that sweep's rotations i = m-1 .. l in cs/sn[buf]; returns false
+// next sweep and stores that sweep's rotations i = m-1 .. l in rot[buf]; returns false
 // when every eigenvalue has converged (or the iteration limit was hit, ql.fail = 1).
-// Runs on all 64 lanes of one wavefront: the state is uniform, the deflation scan and the
-// shift of d are split over the lanes, and the rotation chain runs on lane 0 with the
-// next d[i], e[i] loaded one step ahead.  d, e and the state live in LDS.
+// Runs on all 64 lanes of one wavefront: the state is uniform, the deflation scan, the
+// shift of d and the range check of the sweep are split over the lanes, and the rotation
+// chain runs on lane 0.  d, e and the state live in LDS.
 template <typename R, int NMAX>
 __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
                               unsigned long long& n_rot) {
   QlState<R>& q = sh.ql;
-  R* d = sh.dd;
-  R* e = sh.ee;
+  RPair<R>* de = sh.de;
   const R eps = eps_r<R>();
   int l = q.l, m = q.m, iter = q.iter, in_iter = q.in_iter;
   R f = q.f, tst1 = q.tst1;
   bool more = false;
   while (l < n) {
     if (!in_iter) {
-      tst1 = fmax(tst1, fabs(d[l]) + fabs(e[l]));
+      tst1 = fmax(tst1, fabs(de[l].x) + fabs(de[l].y));
       // m = first index >= l with |e[m]| <= eps tst1 (NaN stops the scan), else n - 1
       int mm = n - 1;
       for (int b = l; b < n - 1; b += 64) {
         const int k = b + lane;
-        const bool small = k < n - 1 && !(fabs(e[k]) > eps * tst1);
+        const bool small = k < n - 1 && !(fabs(de[k].y) > eps * tst1);
         const unsigned long long bal = __ballot(small);
         if (bal) {
           mm = b + __ffsll(static_cast<long long>(bal)) - 1;
@@ -478,8 +586,8 @@ __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
       m = mm;
       if (m == l) {
         if (lane == 0) {
-          d[l] += f;
-          e[l] = 0;
+          de[l].x += f;
+          de[l].y = 0;
         }
         wave_sync();
         ++l;
@@ -494,70 +602,65 @@ __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
       break;
     }
     // shift from the leading 2x2 block (uniform over the lanes)
-    const R g0 = d[l];
-    const R el = e[l];
-    R pp = (d[l + 1] - g0) / (R(2) * el);
+    const R g0 = de[l].x;
+    const R el = de[l].y;
+    R pp = (de[l + 1].x - g0) / (R(2) * el);
     R r = sqrt(pp * pp + R(1));
     if (pp < 0) r = -r;
     const R dl = el / (pp + r);
     const R dl1 = el * (pp + r);
     const R h0 = g0 - dl;
-    for (int i = l + 2 + lane; i < n; i += 64) d[i] -= h0;
+    // shift d[l+2:n) and bound the sweep's operands: max |d_i| + |e_i| over [l, m] and
+    // min |e_i| over [l, m) (NaN fails the check below)
+    R big = 0, small = std::numeric_limits<R>::infinity();
+    for (int i = l + lane; i < n; i += 64) {
+      RPair<R> v = de[i];
+      if (i >= l + 2) {
+        v.x -= h0;
+        de[i].x = v.x;
+      } else {
+        v.x = i == l ? dl : dl1;
+      }
+      if (i <= m) big = fmax(big, fabs(v.x) + fabs(v.y));
+      if (i < m) small = fmin(small, fabs(v.y));
+      if (!(v.x == v.x) || !(v.y == v.y)) big = std::numeric_limits<R>::infinity();
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      big = fmax(big, __shfl_xor(big, off, 64));
+      small = fmin(small, __shfl_xor(small, off, 64));
+    }
+    const bool fast = big <= chain_hi<R>() && small >= chain_lo<R>();
     f += h0;
     wave_sync();  // lane 0 reads the shifted d below
     int conv = 0;
     if (lane == 0) {
-      d[l] = dl;
-      d[l + 1] = dl1;
-      pp = d[m];
-      R c = 1, c2 = 1, c3 = 1, s = 0, s2 = 0;
-      const R el1 = e[l + 1];
-      R* cs = sh.cs[buf];
-      R* sn = sh.sn[buf];
-      R ei = e[m - 1], di = d[m - 1];
-      for (int i = m - 1; i >= l; --i) {
-        // operands of the next step (not written by this sweep before they are used),
-        // loaded unconditionally from an in-range index so that no branch separates the
-        // loads from their use one step later
-        const int ip = i > l ? i - 1 : i;
-        const R ein = e[ip];
-        const R din = d[ip];
-        c3 = c2;
-        c2 = c;
-        s2 = s;
-        const R g = c * ei;
-        const R h = c * pp;
-        // Givens rotation zeroing e[i]: r = sqrt(pp^2 + e_i^2), c = pp / r, s = e_i / r, and
-        // the next pp = c d_i - s g = (pp d_i - e_i g) / r, whose numerator is formed while
-        // the square root is refined (the dependency chain runs through r only); r = 0
-        // (pp = e_i = 0, or below the normal range) is the identity rotation
-        const R x = fma(pp, pp, ei * ei);
-        const R wnum = fma(pp, di, -(ei * g));
-        R rr, hr;
-        sqrt_half_rsqrt(x, rr, hr);
-        const bool ok = x >= min_normal<R>();
-        const R rinv = hr + hr;
-        e[i + 1] = ok ? s * rr : R(0);
-        s = ok ? ei * rinv : R(0);
-        c = ok ? pp * rinv : R(1);
-        pp = ok ? wnum * rinv : di;
-        d[i + 1] = h + s * (c * g + s * di);
-        cs[i] = c;
-        sn[i] = s;
-        ei = ein;
-        di = din;
-      }
+      de[l].x = dl;
+      de[l + 1].x = dl1;
+      pp = de[m].x;
+      R c = 1, s = 0;
+      const R el1 = de[l + 1].y;
+      RPair<R>* rot = sh.rot[buf];
+      if (fast)
+        ql_chain<R, false>(l, m, pp, c, s, de, rot);
+      else
+        ql_chain<R, true>(l, m, pp, c, s, de, rot);
+      // s and c of steps l+1 and l+2 (EISPACK's s2 and c3), 0 and 1 where absent
+      const R s2 = m - 1 >= l + 1 ? rot[l + 1].y : R(0);
+      const R c3 = m - 1 >= l + 2 ? rot[l + 2].x : R(1);
       pp = -s * s2 * c3 * el1 * el / dl1;
-      e[l] = s * pp;
-      d[l] = c * pp;
+      RPair<R> o;
+      o.x = c * pp;
+      o.y = s * pp;
       sh.sweep[buf][0] = l;
       sh.sweep[buf][1] = m;
-      if (!(fabs(e[l]) > eps * tst1)) {
+      if (!(fabs(o.y) > eps * tst1)) {
         // converged: this sweep is the last one for l
-        d[l] += f;
-        e[l] = 0;
+        o.x += f;
+        o.y = 0;
         conv = 1;
       }
+      de[l] = o;
       n_rot += static_cast<unsigned long long>(m - l);
     }
     conv = __shfl(conv, 0);
@@ -689,8 +792,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
         },
         X, ldx, sh);
     __syncthreads();
-    // A = X W^H on and below the diagonal ((W^H)[k, j] = conj(W[j, k]), k <= j), then the
-    // strict upper triangle mirrored from the lower one and a real diagonal
+    // A = X W^H on and below the diagonal ((W^H)[k, j] = conj(W[j, k]), k <= j; the
+    // tridiagonalisation reads the lower triangle only), then a real diagonal
     gemm_tiles<R, NMAX, false, false, true>(
         n, [=](int i, int k) { return X[i + (size_t)k * ldx]; },
         [=](int k, int j) { return k <= j ? conj(W[j + (size_t)k * ld]) : mk<R>(0, 0); },
@@ -700,13 +803,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
         },
         A, ld, sh);
     __syncthreads();
-    for (int j = wave; j < n; j += NW_)
-      for (int i = lane; i <= j && i < n; i += 64) {
-        if (i < j)
-          A[i + (size_t)j * ld] = conj(A[j + (size_t)i * ld]);
-        else
-          A[i + (size_t)i * ld].im = 0;
-      }
+    for (int i = tid; i < n; i += NT_) A[i + (size_t)i * ld].im = 0;
     __syncthreads();
   }
   eig_mark(p, sid, 3);
@@ -714,8 +811,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   // ---- Householder tridiagonalisation (zhetd2, lower, fused her2 + hemv) --------------
   if (n == 1) {
     if (tid == 0) {
-      sh.dd[0] = A[0].re;
-      sh.ee[0] = 0;
+      sh.de[0].x = A[0].re;
+      sh.de[0].y = 0;
     }
     __syncthreads();
   } else {
@@ -723,7 +820,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     cplx<R>* vnext = sh.vn;
     cplx<R> tau = reflector<R, NMAX>(n, 0, A, ld, vcur, sh);
     if (tid == 0) {
-      sh.dd[0] = A[0].re;
+      sh.de[0].x = A[0].re;
       A[0] = tau;  // diagonal slot keeps tau_0 for the back-transform
     }
     trail_pass<R, NMAX, false>(n, 1, A, ld, vcur, vcur, sh);
@@ -737,7 +834,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
         for (int i = k + 1 + tid; i < n; i += NT_) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
       }
       __syncthreads();
-      if (tid == 0) sh.dd[k + 1] = A[(k + 1) + (size_t)(k + 1) * ld].re;
+      if (tid == 0) sh.de[k + 1].x = A[(k + 1) + (size_t)(k + 1) * ld].re;
       tau = reflector<R, NMAX>(n, k + 1, A, ld, vnext, sh);
       if (tid == 0) A[(k + 1) + (size_t)(k + 1) * ld] = tau;
       trail_pass<R, NMAX, true>(n, k + 2, A, ld, vcur, vnext, sh);
@@ -750,9 +847,9 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
       cplx<R> a = A[(n - 1) + (size_t)(n - 1) * ld];
       const cplx<R> vv = vcur[n - 1], ww = sh.w[n - 1];
       a = a - (vv * conj(ww) + ww * conj(vv));
-      sh.dd[n - 1] = a.re;
+      sh.de[n - 1].x = a.re;
       A[(n - 1) + (size_t)(n - 1) * ld] = mk<R>(0, 0);  // no reflector for the last column
-      sh.ee[n - 1] = 0;
+      sh.de[n - 1].y = 0;
     }
     __syncthreads();
   }
@@ -765,7 +862,11 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
   static_assert(NW_ >= 2, "a generator wavefront besides the row owners");
   const int gwave = NW_ - 1;
-  constexpr int NRT = NT_ - 64;  // row-owner threads
+  // wavefronts are placed on the 4 SIMDs in turn: with 8 of them, wave gwave - 4 shares the
+  // generator's SIMD and owns no rows, so that the chain has its SIMD's issue slots
+  const int sib = NW_ >= 8 ? gwave - 4 : -1;
+  constexpr int NRT = NT_ - (NW_ >= 8 ? 128 : 64);  // row-owner threads
+  const int rrank = wave < sib || sib < 0 ? tid : tid - 64;
   unsigned long long gen_cycles = 0, n_rot = 0, n_sweep = 0;
   if (wave == gwave) {
     if (lane == 0) {
@@ -801,9 +902,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     // fetched CH at a time, the loads of the next chunk issued before the current chunk
     // is rotated and stored (the chunks cover disjoint columns)
     constexpr int CH = 24;
-    const R* cs = sh.cs[cur];
-    const R* sn = sh.sn[cur];
-    for (int row = (sweep_ok && wave != gwave) ? tid : n; row < n; row += NRT) {
+    const RPair<R>* rot = sh.rot[cur];
+    for (int row = (sweep_ok && wave != gwave && wave != sib) ? rrank : n; row < n; row += NRT) {
       R* zr = Z + row;
       R carry = zr[(size_t)m * ld];
       R za[CH], zb[CH];
@@ -817,7 +917,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
         for (int u = 0; u < CH; ++u) {
           const int i = i0 - u;
           if (i >= l) {
-            const R c = cs[i], s = sn[i];
+            const RPair<R> cs = rot[i];
+            const R c = cs.x, s = cs.y;
             zr[(size_t)(i + 1) * ld] = s * buf[u] + c * carry;
             carry = c * buf[u] - s * carry;
           }
@@ -851,18 +952,18 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   // ---- ascending order: rank sort --------------------------------------------------
   // (NaN ordered as +inf, ties by index: the ranks always form a permutation of [0, n))
   for (int i = tid; i < n; i += NT_) {
-    const R di0 = sh.dd[i];
+    const R di0 = sh.de[i].x;
     const R di = di0 == di0 ? di0 : std::numeric_limits<R>::infinity();
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const R dj0 = sh.dd[j];
+      const R dj0 = sh.de[j].x;
       const R dj = dj0 == dj0 ? dj0 : std::numeric_limits<R>::infinity();
       rank += (dj < di) || (dj == di && j < i);
     }
     sh.perm[rank] = i;
   }
   __syncthreads();
-  for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.dd[sh.perm[j]] : R(0);
+  for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.de[sh.perm[j]].x : R(0);
 
   // ---- back-transform X = Q Z[:, perm] ------------------------------------------------
   // Each group of LPC lanes keeps one column of Z[:, perm] in registers (rows g + LPC r)
