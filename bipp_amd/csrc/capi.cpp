@@ -435,18 +435,59 @@ int next_smooth(double x) {
   }
 }
 
-N3Radix radices_of(int m) {
+// Stockham stages for length m, radices at most `max_radix` (16, 9, 8 and 6 are
+// in-register composite DFTs; 4, 2, 3, 5 otherwise): large radices first take the factors
+// 2 and 3, then the rest; the stages run in ascending radix order.
+int radix_cap(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  const int x = e ? std::atoi(e) : dflt;
+  return x >= 16 ? 16 : (x >= 9 ? 9 : (x >= 8 ? 8 : (x >= 6 ? 6 : 4)));
+}
+// contiguous-axis lines (BIPP_FFT_RADIX_MAX, default 9) and strided passes
+// (BIPP_FFT_PASS_RADIX_MAX, default 4); the defaults were the fastest at configuration 4
+// (x lines of 3456 as radices 6, 8, 8, 9: 20.9 ms for the FFT against 22.4 ms with radices
+// up to 4; composite radices in the strided passes cost occupancy and were slower)
+int fft_radix_max() {
+  static const int v = radix_cap("BIPP_FFT_RADIX_MAX", 9);
+  return v;
+}
+int fft_pass_radix_max() {
+  static const int v = radix_cap("BIPP_FFT_PASS_RADIX_MAX", 4);
+  return v;
+}
+
+N3Radix radices_of(int m, int max_radix) {
   N3Radix r;
-  while (m % 4 == 0) {
-    r.r[r.count++] = 4;
-    m /= 4;
+  int twos = 0, threes = 0;
+  while (m % 2 == 0) {
+    ++twos;
+    m /= 2;
   }
-  for (int p : {2, 3, 5})
-    while (m % p == 0) {
-      r.r[r.count++] = p;
-      m /= p;
-    }
-  if (m != 1 || r.count > 20) fail(BIPP_INTERNAL_ERROR, "FFT length is not 2^a 3^b 5^c");
+  while (m % 3 == 0) {
+    ++threes;
+    m /= 3;
+  }
+  auto push = [&](int p) {
+    if (r.count >= 20) fail(BIPP_INTERNAL_ERROR, "FFT length has too many factors");
+    r.r[r.count++] = p;
+  };
+  if (max_radix >= 16)
+    for (; twos >= 4; twos -= 4) push(16);
+  if (max_radix >= 8)
+    for (; twos >= 3; twos -= 3) push(8);
+  if (max_radix >= 9)
+    for (; threes >= 2; threes -= 2) push(9);
+  if (max_radix >= 6)
+    for (; twos >= 1 && threes >= 1; --twos, --threes) push(6);
+  for (; twos >= 2; twos -= 2) push(4);
+  if (twos == 1) push(2);
+  for (; threes >= 1; --threes) push(3);
+  while (m % 5 == 0) {
+    push(5);
+    m /= 5;
+  }
+  if (m != 1) fail(BIPP_INTERNAL_ERROR, "FFT length is not 2^a 3^b 5^c");
+  std::sort(r.r, r.r + r.count);
   return r;
 }
 
@@ -765,7 +806,7 @@ struct N3Runner {
     lp.zw = pl.in_w[2];
     lp.in = pl.in_w[0];
     lp.out = pl.out_w[0];
-    lp.rad = radices_of(nx);
+    lp.rad = radices_of(nx, fft_radix_max());
     lp.tw = dtw + tw_off[0][0];
     launch_fft_lines<TG>(lp, st);
     // y over (x of the output window, z of the input window); z over (x, y output windows)
@@ -806,7 +847,7 @@ struct N3Runner {
         fp.fb = 1;
         fp.twiddle = 0;
         fp.ntot = n1;
-        fp.rad = radices_of(n1);
+        fp.rad = radices_of(n1, fft_pass_radix_max());
         fp.tw = dtw + tw_off[d][0];
         fp.tw2 = nullptr;
         launch_fft_pass<TG>(fp, st);
@@ -822,7 +863,7 @@ struct N3Runner {
         fp.out_check = 0;
         fp.twiddle = 1;
         fp.ntot = (long long)n1 * n2;
-        fp.rad = radices_of(n1);
+        fp.rad = radices_of(n1, fft_pass_radix_max());
         fp.tw = dtw + tw_off[d][0];
         fp.tw2 = dtw + tw_off[d][2];
         launch_fft_pass<TG>(fp, st);
@@ -835,7 +876,7 @@ struct N3Runner {
         fp.fa = 1;
         fp.fb = n1;
         fp.twiddle = 0;
-        fp.rad = radices_of(n2);
+        fp.rad = radices_of(n2, fft_pass_radix_max());
         fp.tw = dtw + tw_off[d][1];
         fp.tw2 = nullptr;
         launch_fft_pass<TG>(fp, st);

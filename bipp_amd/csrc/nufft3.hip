@@ -376,6 +376,72 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
 // Stockham FFT in LDS (sign +): `nl` lines of length n, line stride ld, radices r[].
 // Returns the buffer holding the result.
 // --------------------------------------------------------------------------------------
+// exp(+2 pi i m / N) for the constant twiddles of the composite radices (m is a
+// compile-time constant after unrolling, so the selection folds away).
+template <typename R, int N>
+__device__ __forceinline__ cplx<R> root_const(int m) {
+  constexpr double C8 = 0.923879532511286756128, S8 = 0.382683432365089771728,
+                   H2 = 0.707106781186547524401, H3 = 0.866025403784438646764;
+  double c = 1, s = 0;
+  if constexpr (N == 16) {
+    constexpr double tc[10] = {1, C8, H2, S8, 0, -S8, -H2, -C8, -1, -C8};
+    constexpr double ts[10] = {0, S8, H2, C8, 1, C8, H2, S8, 0, -S8};
+    c = tc[m];
+    s = ts[m];
+  } else if constexpr (N == 9) {
+    constexpr double tc[5] = {1, 0.766044443118978035202, 0.173648177666930348852, -0.5,
+                              -0.939692620785908384054};
+    constexpr double ts[5] = {0, 0.642787609686539326323, 0.984807753012208059367, H3,
+                              0.342020143325668733044};
+    c = tc[m];
+    s = ts[m];
+  } else if constexpr (N == 8) {
+    constexpr double tc[4] = {1, H2, 0, -H2};
+    constexpr double ts[4] = {0, H2, 1, H2};
+    c = tc[m];
+    s = ts[m];
+  } else {
+    static_assert(N == 6, "roots of the composite radices");
+    constexpr double tc[3] = {1, 0.5, -0.5};
+    constexpr double ts[3] = {0, H3, H3};
+    c = tc[m];
+    s = ts[m];
+  }
+  return mk<R>(static_cast<R>(c), static_cast<R>(s));
+}
+
+template <typename R, int RAD>
+__device__ __forceinline__ void dft_small(cplx<R>* v);
+
+// DFT of length N = P Q (sign +) in registers, Cooley-Tukey: input j = Q j1 + j2, output
+// k = k1 + P k2, X[k] = sum_j2 w_N^(j2 k1) w_Q^(j2 k2) sum_j1 x[Q j1 + j2] w_P^(j1 k1).
+template <typename R, int P, int Q>
+__device__ __forceinline__ void dft_ct(cplx<R>* v) {
+  constexpr int N = P * Q;
+  cplx<R> f[N];
+#pragma unroll
+  for (int j2 = 0; j2 < Q; ++j2) {
+    cplx<R> t[P];
+#pragma unroll
+    for (int j1 = 0; j1 < P; ++j1) t[j1] = v[Q * j1 + j2];
+    dft_small<R, P>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      const int m = (j2 * k1) % N;
+      f[j2 * P + k1] = m == 0 ? t[k1] : t[k1] * root_const<R, N>(m);
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) {
+    cplx<R> t[Q];
+#pragma unroll
+    for (int j2 = 0; j2 < Q; ++j2) t[j2] = f[j2 * P + k1];
+    dft_small<R, Q>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = t[k2];
+  }
+}
+
 // Small DFTs with sign +, constant roots.
 template <typename R, int RAD>
 __device__ __forceinline__ void dft_small(cplx<R>* v) {
@@ -400,8 +466,16 @@ __device__ __forceinline__ void dft_small(cplx<R>* v) {
     v[0] = a + t1;
     v[1] = m + q;
     v[2] = m - q;
+  } else if constexpr (RAD == 6) {
+    dft_ct<R, 2, 3>(v);
+  } else if constexpr (RAD == 8) {
+    dft_ct<R, 2, 4>(v);
+  } else if constexpr (RAD == 9) {
+    dft_ct<R, 3, 3>(v);
+  } else if constexpr (RAD == 16) {
+    dft_ct<R, 4, 4>(v);
   } else {
-    static_assert(RAD == 5, "radix 2, 3, 4 or 5");
+    static_assert(RAD == 5, "radix 2, 3, 4, 5, 6, 8, 9 or 16");
     // radix 5: roots exp(+2 pi i k / 5)
     const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
     const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
@@ -454,10 +528,27 @@ __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n
     cplx<R>* dst = out + (size_t)line * ld;
     const int jq = j / ns;
     const int k = j - jq * ns;
+    // twiddles w^r, w = exp(2 pi i k tstride / n): w, w^4 and w^8 from the table, the
+    // others as products at most two deep (k r tstride < n for r < RAD)
+    const int t1 = k * tstride;
+    cplx<R> wp[RAD];
+    if constexpr (RAD > 1) wp[1] = tw(t1);
+    if constexpr (RAD > 2) wp[2] = wp[1] * wp[1];
+    if constexpr (RAD > 3) wp[3] = wp[2] * wp[1];
+    if constexpr (RAD > 4) {
+      wp[4] = tw(4 * t1);
+#pragma unroll
+      for (int r = 5; r < RAD && r < 8; ++r) wp[r] = wp[4] * wp[r - 4];
+    }
+    if constexpr (RAD > 8) {
+      wp[8] = tw(8 * t1);
+#pragma unroll
+      for (int r = 9; r < RAD; ++r) wp[r] = wp[8] * wp[r - 8];
+    }
     cplx<R> v[RAD];
     v[0] = src[j];
 #pragma unroll
-    for (int r = 1; r < RAD; ++r) v[r] = src[j + r * nb] * tw(k * r * tstride);
+    for (int r = 1; r < RAD; ++r) v[r] = src[j + r * nb] * wp[r];
     dft_small<R, RAD>(v);
     const int idx = jq * ns * RAD + k;
 #pragma unroll
@@ -465,8 +556,11 @@ __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n
   }
 }
 
-// Stockham stages in LDS; returns the buffer holding the result.
-template <typename R>
+// Stockham stages in LDS; returns the buffer holding the result.  MAXR: the largest radix
+// compiled in (5: radices 2-5 only, which keeps the kernel's register and scalar-register
+// footprint small; 16: also the composite radices 6, 8, 9, 16); the launchers pick it from
+// the plan.
+template <typename R, int MAXR>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
                             const Twid<R>& tw, int tid, int nt) {
   int ns = 1;
@@ -478,6 +572,18 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
       case 2: fft_stage<R, 2>(in, out, n, nl, ld, ns, tw, tid, nt); break;
       case 3: fft_stage<R, 3>(in, out, n, nl, ld, ns, tw, tid, nt); break;
       case 4: fft_stage<R, 4>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      case 6:
+        if constexpr (MAXR >= 6) fft_stage<R, 6>(in, out, n, nl, ld, ns, tw, tid, nt);
+        break;
+      case 8:
+        if constexpr (MAXR >= 8) fft_stage<R, 8>(in, out, n, nl, ld, ns, tw, tid, nt);
+        break;
+      case 9:
+        if constexpr (MAXR >= 9) fft_stage<R, 9>(in, out, n, nl, ld, ns, tw, tid, nt);
+        break;
+      case 16:
+        if constexpr (MAXR >= 16) fft_stage<R, 16>(in, out, n, nl, ld, ns, tw, tid, nt);
+        break;
       default: fft_stage<R, 5>(in, out, n, nl, ld, ns, tw, tid, nt); break;
     }
     __syncthreads();
@@ -502,7 +608,7 @@ constexpr int FFT_PF = 8;
 
 // Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).  The
 // next group's inputs are fetched into registers while the current group is transformed.
-template <typename R>
+template <typename R, int MAXR>
 __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, int nl) {
   extern __shared__ __align__(16) unsigned char fsh[];
   __shared__ long long lbase[2][16];
@@ -557,7 +663,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
     if (gn < n_groups) set_bases(gn, cb ^ 1);
     __syncthreads();
     if (gn < n_groups) fetch(gn, cb ^ 1, nxt);  // in flight during the transform
-    cplx<R>* res = lds_fft<R>(a, b, n, nlg, n, p.rad, tw, tid, FFT_NT);
+    cplx<R>* res = lds_fft<R, MAXR>(a, b, n, nlg, n, p.rad, tw, tid, FFT_NT);
     for (int i = tid; i < nlg * n; i += FFT_NT) {
       const int li = i / n, k = i - li * n;
       if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
@@ -573,7 +679,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
 // sub-transform `sub` at axis index sub * sub_stride + i * estride.  The LDS line stride is
 // odd so that the TW lines of a load fall into different banks.  (Latency is hidden by
 // occupancy: a register prefetch of the next item costs more waves than it saves.)
-template <typename R>
+template <typename R, int MAXR>
 __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int tw_shift) {
   extern __shared__ __align__(16) unsigned char fsh[];
   const int TW = 1 << tw_shift;  // x positions per item (power of two)
@@ -614,7 +720,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
       a[xi * ld + i] = v;
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R>(a, b, p.m, TW, ld, p.rad, tw, threadIdx.x, blockDim.x);
+    cplx<R>* res = lds_fft<R, MAXR>(a, b, p.m, TW, ld, p.rad, tw, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
       const int xi = t & (TW - 1), k = t >> tw_shift;
       const int j = (xt << tw_shift) + xi;
@@ -864,6 +970,11 @@ namespace {
 bool window_ok(const N3Window& w, int n) {
   return n > 0 && w.start >= 0 && w.start < n && w.count >= 0 && w.count <= n;
 }
+inline int max_radix(const N3Radix& r) {
+  int m = 0;
+  for (int i = 0; i < r.count; ++i) m = std::max(m, r.r[i]);
+  return m;
+}
 }  // namespace
 
 template <typename R>
@@ -879,15 +990,21 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
   const size_t sh = sizeof(cplx<R>) * (2 * (size_t)p.n * nl + twid_len(p.n));
   // 1 KiB of the 160 KiB stays for the kernel's static line bases
   if (sh > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
-  static bool attr_set = false;
-  if (!attr_set) {
-    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_lines_kernel<R>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
-    attr_set = true;
-  }
   const long long groups = (n_lines + nl - 1) / nl;
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(groups, 32768));
-  fft_lines_kernel<R><<<blocks, FFT_NT, sh, st>>>(p, nl);
+  auto go = [&](auto kernel, bool& attr_set) {
+    if (!attr_set) {
+      BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+      attr_set = true;
+    }
+    kernel<<<blocks, FFT_NT, sh, st>>>(p, nl);
+  };
+  static bool attr5 = false, attr16 = false;
+  if (max_radix(p.rad) > 5)
+    go(&fft_lines_kernel<R, 16>, attr16);
+  else
+    go(&fft_lines_kernel<R, 5>, attr5);
   check_launch("fft_lines_kernel");
 }
 template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
@@ -912,14 +1029,20 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
   const size_t sh = sizeof(cplx<R>) * (2 * (size_t)TW * ld + twid_len(p.m) +
                                        (p.twiddle ? twid_len(static_cast<int>(p.ntot)) : 0));
   if (sh > 160 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");
-  static bool attr_set = false;
-  if (!attr_set) {
-    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_pass_kernel<R>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 32768));
-  fft_pass_kernel<R><<<blocks, FFT_NT, sh, st>>>(p, tw_shift);
+  auto go = [&](auto kernel, bool& attr_set) {
+    if (!attr_set) {
+      BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      attr_set = true;
+    }
+    kernel<<<blocks, FFT_NT, sh, st>>>(p, tw_shift);
+  };
+  static bool attr5 = false, attr16 = false;
+  if (max_radix(p.rad) > 5)
+    go(&fft_pass_kernel<R, 16>, attr16);
+  else
+    go(&fft_pass_kernel<R, 5>, attr5);
   check_launch("fft_pass_kernel");
 }
 template void launch_fft_pass<float>(const N3PassParams<float>&, hipStream_t);
