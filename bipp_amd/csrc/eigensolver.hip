@@ -127,44 +127,6 @@ struct EigShared {
 };
 
 // --------------------------------------------------------------------------------------
-// B[i, j] -= x[i] * y_j for columns j = j0 + wave + t NW < n and rows i in [rlo(j), rhi):
-// the wave loads UC columns (RM rows per lane each) before it stores any of them.
-// y_j = y[j], or conj(y[j]) with CONJY.
-// --------------------------------------------------------------------------------------
-template <typename R, int NMAX, bool CONJY, typename RowLo>
-__device__ __forceinline__ void cols_update(int j0, int n, int rhi, RowLo rlo,
-                                            cplx<R>* __restrict__ B, int ldb, const cplx<R>* x,
-                                            const cplx<R>* y) {
-  constexpr int NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM, UC = EigCfg<NMAX>::UC;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int jb = j0 + wave; jb < n; jb += NW_ * UC) {
-    cplx<R> val[UC][RM];
-#pragma unroll
-    for (int u = 0; u < UC; ++u) {
-      const int j = jb + u * NW_;
-      const int lo = j < n ? rlo(j) : rhi;
-#pragma unroll
-      for (int r = 0; r < RM; ++r) {
-        const int i = lane + 64 * r;
-        if (i >= lo && i < rhi) val[u][r] = B[i + (size_t)j * ldb];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < UC; ++u) {
-      const int j = jb + u * NW_;
-      if (j >= n) continue;
-      const int lo = rlo(j);
-      const cplx<R> yj = CONJY ? conj(y[j]) : y[j];
-#pragma unroll
-      for (int r = 0; r < RM; ++r) {
-        const int i = lane + 64 * r;
-        if (i >= lo && i < rhi) B[i + (size_t)j * ldb] = val[u][r] - x[i] * yj;
-      }
-    }
-  }
-}
-
-// --------------------------------------------------------------------------------------
 // W = L^-1 in place on the lower triangle of L (ztrti2, lower, non-unit): columns from the
 // last to the first, W[j+1:n, j] = -W[j+1:n, j+1:n] L[j+1:n, j] / L[j, j], the product
 // taken over the lower triangle only (the strict upper triangle of L is never read).
@@ -754,12 +716,55 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 1);
 
   if (!p.standard) {
-    // ---- Cholesky G = L L^H (lower, right-looking) -------------------------------------
+    // ---- Cholesky G = L L^H (lower, left-looking) --------------------------------------
+    // column j: y = G[j:n, j] - L[j:n, 0:j] conj(L[j, 0:j])^T (lanes over rows, wavefronts
+    // over the finished columns), L[j, j] = sqrt(y_j), L[j+1:n, j] = y[j+1:n] / L[j, j]:
+    // each finished element is read once per later column and nothing but column j is
+    // written; the strict upper triangle is never read
     for (int j = 0; j < n; ++j) {
-      // every thread reads the pivot (one broadcast load per wave): no barrier needed
-      const R djj = L[j + (size_t)j * ld].re;
-      const bool ok = djj > R(0) && djj == djj;
-      if (!ok) {
+      for (int k = tid; k < j; k += NT_) sh.v[k] = conj(L[j + (size_t)k * ld]);
+      __syncthreads();
+      cplx<R> acc[RM];
+#pragma unroll
+      for (int r = 0; r < RM; ++r) acc[r] = mk<R>(0, 0);
+      for (int kb = wave; kb < j; kb += NW_ * UC) {
+        cplx<R> a[UC][RM];
+#pragma unroll
+        for (int u = 0; u < UC; ++u) {
+          const int k = kb + u * NW_;
+#pragma unroll
+          for (int r = 0; r < RM; ++r) {
+            const int i = j + lane + 64 * r;
+            if (k < j && i < n) a[u][r] = L[i + (size_t)k * ld];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < UC; ++u) {
+          const int k = kb + u * NW_;
+          if (k >= j) continue;
+          const cplx<R> vk = sh.v[k];
+#pragma unroll
+          for (int r = 0; r < RM; ++r) {
+            const int i = j + lane + 64 * r;
+            if (i < n) acc[r] = acc[r] + a[u][r] * vk;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RM; ++r) {
+        const int i = j + lane + 64 * r;
+        if (i < n) sh.xw[wave][i] = acc[r];
+      }
+      __syncthreads();
+      for (int i = j + tid; i < n; i += NT_) {
+        cplx<R> y = L[i + (size_t)j * ld];
+#pragma unroll
+        for (int w = 0; w < NW_; ++w) y = y - sh.xw[w][i];
+        sh.w[i] = y;
+      }
+      __syncthreads();
+      const R djj = sh.w[j].re;  // every thread reads the same pivot
+      if (!(djj > R(0) && djj == djj)) {
         // leading minor j+1 of G is not positive definite: ?hegv reports info = n + j + 1
         for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
         if (tid == 0) p.info[sid] = n + j + 1;
@@ -768,15 +773,9 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
       }
       const R sj = sqrt(djj);
       const R inv = R(1) / sj;
-      for (int i = j + 1 + tid; i < n; i += NT_) {
-        cplx<R> u = L[i + (size_t)j * ld] * inv;
-        L[i + (size_t)j * ld] = u;
-        sh.v[i] = u;
-      }
-      __syncthreads();  // every thread has read the pivot; sh.v complete
+      for (int i = j + 1 + tid; i < n; i += NT_) L[i + (size_t)j * ld] = sh.w[i] * inv;
       if (tid == 0) L[j + (size_t)j * ld] = mk<R>(sj, 0);
-      cols_update<R, NMAX, true>(j + 1, n, n, [](int kk) { return kk; }, L, ld, sh.v, sh.v);
-      __syncthreads();
+      __syncthreads();  // column j complete before the next column stages row j+1
     }
   eig_mark(p, sid, 2);
     // ---- C = W S W^H with W = L^-1 -----------------------------------------------------
