@@ -116,13 +116,13 @@ struct EigShared {
     BackTile<R, NMAX> bt;   // reflectors of the back-transform
   };
   RPair<R> de[NMAX];      // tridiagonal: (diagonal d_i, subdiagonal e_i)
-  RPair<R> rot[2][NMAX];  // rotations (c_i, s_i) of two QL sweeps
+  RPair<R> rot[2][2][NMAX];  // rotations (c_i, s_i) of two pairs of QL sweeps
   int perm[NMAX];
   int idx[NMAX];
   R red[2 * NW_];
   R scal[4];
   int iscal[4];
-  int sweep[2][3];  // (l, m, valid) of the rotation buffers
+  int sweep[2][2][3];  // (l, m, valid) of the rotation buffers
   QlState<R> ql;
 };
 
@@ -517,14 +517,15 @@ __device__ __forceinline__ void ql_chain(int l, int m, R& pp, R& c, R& s, RPair<
 }
 
 // Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
-// next sweep and stores that sweep's rotations i = m-1 .. l in rot[buf]; returns false
+// next sweep and stores that sweep's rotations i = m-1 .. l in rot_out and (l, m) in sw;
+// returns false
 // when every eigenvalue has converged (or the iteration limit was hit, ql.fail = 1).
 // Runs on all 64 lanes of one wavefront: the state is uniform, the deflation scan, the
 // shift of d and the range check of the sweep are split over the lanes, and the rotation
 // chain runs on lane 0.  d, e and the state live in LDS.
 template <typename R, int NMAX>
-__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
-                              unsigned long long& n_rot) {
+__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, RPair<R>* rot_out, int* sw,
+                              int lane, unsigned long long& n_rot) {
   QlState<R>& q = sh.ql;
   RPair<R>* de = sh.de;
   const R eps = eps_r<R>();
@@ -602,7 +603,7 @@ __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
       pp = de[m].x;
       R c = 1, s = 0;
       const R el1 = de[l + 1].y;
-      RPair<R>* rot = sh.rot[buf];
+      RPair<R>* rot = rot_out;
       if (fast)
         ql_chain<R, false>(l, m, pp, c, s, de, rot);
       else
@@ -614,8 +615,8 @@ __device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, int buf, int lane,
       RPair<R> o;
       o.x = c * pp;
       o.y = s * pp;
-      sh.sweep[buf][0] = l;
-      sh.sweep[buf][1] = m;
+      sw[0] = l;
+      sw[1] = m;
       if (!(fabs(o.y) > eps * tst1)) {
         // converged: this sweep is the last one for l
         o.x += f;
@@ -855,8 +856,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 4);
 
   // ---- implicit QL on the real tridiagonal (tql2), rotations into Z -------------------
-  // The last wavefront produces the rotations of sweep t+1 while the other threads apply
-  // those of sweep t to their rows of Z (double-buffered).
+  // The last wavefront produces the rotations of the next pair of sweeps while the other
+  // threads apply the current pair to their rows of Z (double-buffered pairs).
   for (int j = wave; j < n; j += NW_)
     for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
   static_assert(NW_ >= 2, "a generator wavefront besides the row owners");
@@ -867,6 +868,18 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   constexpr int NRT = NT_ - (NW_ >= 8 ? 128 : 64);  // row-owner threads
   const int rrank = wave < sib || sib < 0 ? tid : tid - 64;
   unsigned long long gen_cycles = 0, n_rot = 0, n_sweep = 0;
+  // up to two sweeps into pair buffer pb (the second only while the iteration goes on)
+  auto gen_pair = [&](int pb) {
+    const unsigned long long t0 = p.prof ? wall_clock64() : 0;
+    const bool more = ql_next_sweep<R, NMAX>(n, sh, sh.rot[pb][0], sh.sweep[pb][0], lane, n_rot);
+    const bool more2 =
+        more ? ql_next_sweep<R, NMAX>(n, sh, sh.rot[pb][1], sh.sweep[pb][1], lane, n_rot) : false;
+    if (lane == 0) {
+      sh.sweep[pb][0][2] = more ? 1 : 0;
+      sh.sweep[pb][1][2] = more2 ? 1 : 0;
+    }
+    if (p.prof) gen_cycles += wall_clock64() - t0;
+  };
   if (wave == gwave) {
     if (lane == 0) {
       sh.ql.f = 0;
@@ -878,60 +891,73 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
       sh.ql.fail = 0;
     }
     wave_sync();
-    const unsigned long long t0 = p.prof ? wall_clock64() : 0;
-    const bool more = ql_next_sweep<R, NMAX>(n, sh, 0, lane, n_rot);
-    if (p.prof) gen_cycles += wall_clock64() - t0;
-    if (lane == 0) sh.sweep[0][2] = more ? 1 : 0;
+    gen_pair(0);
   }
   __syncthreads();
   for (int t = 0;; ++t) {
     const int cur = t & 1;
-    if (!sh.sweep[cur][2]) break;
-    const int l = sh.sweep[cur][0], m = sh.sweep[cur][1];
-    ++n_sweep;
-    if (wave == gwave) {
-      const unsigned long long t0 = p.prof ? wall_clock64() : 0;
-      const bool more = ql_next_sweep<R, NMAX>(n, sh, cur ^ 1, lane, n_rot);
-      if (p.prof) gen_cycles += wall_clock64() - t0;
-      if (lane == 0) sh.sweep[cur ^ 1][2] = more ? 1 : 0;
-    }
+    if (!sh.sweep[cur][0][2]) break;
+    const int lA = sh.sweep[cur][0][0], mA = sh.sweep[cur][0][1];
+    const bool vB = sh.sweep[cur][1][2] != 0;
+    const int lB = vB ? sh.sweep[cur][1][0] : 0, mB = vB ? sh.sweep[cur][1][1] : 0;
+    n_sweep += vB ? 2 : 1;
+    if (wave == gwave) gen_pair(cur ^ 1);
     // a sweep always satisfies 0 <= l < m < n; anything else is skipped, never indexed
-    const bool sweep_ok = l >= 0 && l < m && m < n;
-    // apply rotations i = m-1 .. l to the rows of Z owned by this thread; columns are
-    // fetched CH at a time, the loads of the next chunk issued before the current chunk
-    // is rotated and stored (the chunks cover disjoint columns)
+    const bool run = lA >= 0 && lA < mA && mA < n && (!vB || (lB >= 0 && lB < mB && mB < n));
+    const int top = vB ? max(mA, mB) : mA, bot = vB ? min(lA, lB) : lA;
+    const unsigned spanA = static_cast<unsigned>(mA - lA);
+    const unsigned spanB = vB ? static_cast<unsigned>(mB - lB) : 0u;
+    // Each row passes once over columns top .. bot: sweep A's rotation at i (identity
+    // outside [lA, mA)) and, one column behind, sweep B's at i + 1 on A's result (identity
+    // outside [lB, mB)), so a pair of sweeps loads and stores each element once.  Columns
+    // are fetched CH at a time, the next chunk's loads issued before the current chunk is
+    // rotated and stored (stores land at columns above every pending load).
     constexpr int CH = 24;
-    const RPair<R>* rot = sh.rot[cur];
-    for (int row = (sweep_ok && wave != gwave && wave != sib) ? rrank : n; row < n; row += NRT) {
+    const RPair<R>* rotA = sh.rot[cur][0];
+    const RPair<R>* rotB = sh.rot[cur][1];
+    for (int row = (run && wave != gwave && wave != sib) ? rrank : n; row < n; row += NRT) {
       R* zr = Z + row;
-      R carry = zr[(size_t)m * ld];
+      R carryA = zr[(size_t)top * ld];
+      R carryB = 0;
+      auto rotB_at = [&](int k, R zk) {
+        const RPair<R> b = rotB[k];
+        const bool in = static_cast<unsigned>(k - lB) < spanB;
+        const R cb = in ? b.x : R(1), sb = in ? b.y : R(0);
+        zr[(size_t)(k + 1) * ld] = sb * zk + cb * carryB;
+        carryB = cb * zk - sb * carryB;
+      };
+      auto step = [&](int i, R zi) {
+        const RPair<R> a = rotA[i];
+        const bool in = static_cast<unsigned>(i - lA) < spanA;
+        const R ca = in ? a.x : R(1), sa = in ? a.y : R(0);
+        const R e = sa * zi + ca * carryA;  // column i + 1 after sweep A
+        carryA = ca * zi - sa * carryA;
+        if (i == top - 1)
+          carryB = e;  // column top after sweep A starts sweep B's chain
+        else
+          rotB_at(i + 1, e);
+      };
       R za[CH], zb[CH];
       auto fetch = [&](R* buf, int i0) {
 #pragma unroll
         for (int u = 0; u < CH; ++u)
-          if (i0 - u >= l) buf[u] = zr[(size_t)(i0 - u) * ld];
+          if (i0 - u >= bot) buf[u] = zr[(size_t)(i0 - u) * ld];
       };
       auto rotate = [&](const R* buf, int i0) {
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-          const int i = i0 - u;
-          if (i >= l) {
-            const RPair<R> cs = rot[i];
-            const R c = cs.x, s = cs.y;
-            zr[(size_t)(i + 1) * ld] = s * buf[u] + c * carry;
-            carry = c * buf[u] - s * carry;
-          }
-        }
+        for (int u = 0; u < CH; ++u)
+          if (i0 - u >= bot) step(i0 - u, buf[u]);
       };
-      fetch(za, m - 1);
-      for (int i0 = m - 1; i0 >= l; i0 -= 2 * CH) {
-        if (i0 - CH >= l) fetch(zb, i0 - CH);
+      fetch(za, top - 1);
+      for (int i0 = top - 1; i0 >= bot; i0 -= 2 * CH) {
+        if (i0 - CH >= bot) fetch(zb, i0 - CH);
         rotate(za, i0);
-        if (i0 - CH < l) break;
-        if (i0 - 2 * CH >= l) fetch(za, i0 - 2 * CH);
+        if (i0 - CH < bot) break;
+        if (i0 - 2 * CH >= bot) fetch(za, i0 - 2 * CH);
         rotate(zb, i0 - CH);
       }
-      zr[(size_t)l * ld] = carry;
+      rotB_at(bot, carryA);  // sweep B's rotation at bot on column bot after sweep A
+      zr[(size_t)bot * ld] = carryB;
     }
     __syncthreads();
   }
