@@ -472,22 +472,23 @@ __device__ __forceinline__ void ql_step(int i, R ei, R di, R& pp, R& c, R& s, RP
   const R g = c * ei;
   const R h = c * pp;
   const R x = fma(pp, pp, ei * ei);
-  const R wnum = fma(pp, di, -(ei * g));
+  // doubled operands times hr = 1/(2r): the factor 2 stays off the dependency chain
+  const R wnum2 = R(2) * fma(pp, di, -(ei * g));
+  const R pp2 = pp + pp, ei2 = ei + ei;
   R rr, hr;
   sqrt_half_rsqrt(x, rr, hr);
-  const R rinv = hr + hr;
   R enew;
   if (GUARD) {
     const bool ok = x >= min_normal<R>();
     enew = ok ? s * rr : R(0);
-    s = ok ? ei * rinv : R(0);
-    c = ok ? pp * rinv : R(1);
-    pp = ok ? wnum * rinv : di;
+    s = ok ? ei2 * hr : R(0);
+    c = ok ? pp2 * hr : R(1);
+    pp = ok ? wnum2 * hr : di;
   } else {
     enew = s * rr;
-    s = ei * rinv;
-    c = pp * rinv;
-    pp = wnum * rinv;
+    s = ei2 * hr;
+    c = pp2 * hr;
+    pp = wnum2 * hr;
   }
   RPair<R> o;
   o.x = h + s * (c * g + s * di);
