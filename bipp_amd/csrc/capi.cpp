@@ -532,10 +532,17 @@ void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
   }
 }
 
-double es_ft_host(double xi, double beta, const std::vector<double>& x, const std::vector<double>& w) {
+// Quadrature weights times the kernel at the nodes: ew[i] = w[i] exp(beta (sqrt(1 - x_i^2) - 1)),
+// so that phi(xi) = 2 sum_i ew[i] cos(xi x_i).
+std::vector<double> es_ft_weights(double beta, const std::vector<double>& x,
+                                  const std::vector<double>& w) {
+  std::vector<double> ew(x.size());
+  for (size_t i = 0; i < x.size(); ++i) ew[i] = w[i] * std::exp(beta * (std::sqrt(1 - x[i] * x[i]) - 1));
+  return ew;
+}
+double es_ft_host(double xi, const std::vector<double>& x, const std::vector<double>& ew) {
   double s = 0;
-  for (size_t i = 0; i < x.size(); ++i)
-    s += w[i] * std::exp(beta * (std::sqrt(1 - x[i] * x[i]) - 1)) * std::cos(xi * x[i]);
+  for (size_t i = 0; i < x.size(); ++i) s += ew[i] * std::cos(xi * x[i]);
   return 2 * s;
 }
 
@@ -550,8 +557,13 @@ struct N3Plan {
   // FFT windows: spread input l in [-nf1/2, nf1/2) and the cells the pixels interpolate from
   N3Window in_w[3], out_w[3];
   long long grid_cells = 0;
-  std::vector<double> gl_x, gl_w;
+  std::vector<double> gl_x, gl_w, gl_ew;
   std::vector<double> deconv[3];
+  // kernel (w, beta) and grid sizes the tables were last built for (they are reused)
+  int tab_w = -1, dec_nf1[3] = {-1, -1, -1}, dec_nf2[3] = {-1, -1, -1};
+  // phi(xi) = int_{-1}^{1} exp(beta (sqrt(1 - z^2) - 1)) cos(xi z) dz at xi = (j - 1) / ft_inv_delta
+  std::vector<double> ft_tab;
+  double ft_inv_delta = 0;
 
   void make(double tol, const double X_in[3], const double S_in[3], const double C_in[3],
             const double D_in[3], int max_x) {
@@ -559,7 +571,11 @@ struct N3Plan {
     w = std::max(2, std::min(w, 16));
     beta = w == 2 ? 2.20 * w : (w == 3 ? 2.26 * w : (w == 4 ? 2.38 * w : 2.30 * w));
     const double sigma = 2.0;
-    gauss_legendre01(64, gl_x, gl_w);
+    const bool new_kernel = tab_w != w;
+    if (new_kernel) {
+      gauss_legendre01(64, gl_x, gl_w);
+      gl_ew = es_ft_weights(beta, gl_x, gl_w);
+    }
     grid_cells = 1;
     for (int d = 0; d < 3; ++d) {
       const double S = std::max(S_in[d], 1e-9);
@@ -598,12 +614,28 @@ struct N3Plan {
       else
         out_w[d] = N3Window{static_cast<int>(((klo % nf2[d]) + nf2[d]) % nf2[d]),
                             static_cast<int>(cnt)};
-      deconv[d].resize(nf1[d]);
-      const double a2 = 0.5 * w * h2[d];
-      for (int i = 0; i < nf1[d]; ++i) {
-        const int l = i - nf1[d] / 2;
-        deconv[d][i] = 1.0 / (a2 * es_ft_host(l * a2, beta, gl_x, gl_w));
+      if (new_kernel || dec_nf1[d] != nf1[d] || dec_nf2[d] != nf2[d]) {
+        deconv[d].resize(nf1[d]);
+        const double a2 = 0.5 * w * h2[d];
+        for (int i = 0; i < nf1[d]; ++i) {
+          const int l = i - nf1[d] / 2;
+          deconv[d][i] = 1.0 / (a2 * es_ft_host(l * a2, gl_x, gl_ew));
+        }
+        dec_nf1[d] = nf1[d];
+        dec_nf2[d] = nf2[d];
       }
+    }
+    // the pixel-side correction evaluates phi at |xi| = |gam (x - D)| w h1 / 2, at most
+    // pi w / (2 sigma) for pixels inside the extent: tabulated with margin, interpolated
+    // by cubics on the device (phi is even; entry 0 is phi(-delta) = phi(delta))
+    if (new_kernel) {
+      constexpr int NFT = 2048;
+      const double xi_max = 1.05 * 3.14159265358979323846 * w / (2 * sigma) + 1e-3;
+      const double delta = xi_max / NFT;
+      ft_tab.resize(NFT + 3);
+      for (int j = 0; j < NFT + 3; ++j) ft_tab[j] = es_ft_host(std::fabs((j - 1) * delta), gl_x, gl_ew);
+      ft_inv_delta = 1.0 / delta;
+      tab_w = w;
     }
   }
   long long n_bins() const { return (long long)nbins[0] * nbins[1] * nbins[2]; }
@@ -885,10 +917,8 @@ struct N3Runner {
 
     mark(4);
     // interpolation, correction, real part into the images
-    double* dgl = gl.as<double>(2 * pl.gl_x.size());
-    BIPP_HIP(hipMemcpyAsync(dgl, pl.gl_x.data(), sizeof(double) * pl.gl_x.size(),
-                            hipMemcpyHostToDevice, st));
-    BIPP_HIP(hipMemcpyAsync(dgl + pl.gl_x.size(), pl.gl_w.data(), sizeof(double) * pl.gl_w.size(),
+    double* dft = gl.as<double>(pl.ft_tab.size());
+    BIPP_HIP(hipMemcpyAsync(dft, pl.ft_tab.data(), sizeof(double) * pl.ft_tab.size(),
                             hipMemcpyHostToDevice, st));
     N3InterpParams<TG> ip;
     ip.n_pix = n_pix;
@@ -908,9 +938,9 @@ struct N3Runner {
       ip.n2[d] = pl.n2[d];
     }
     ip.lmn = lmn64;
-    ip.gl_x = dgl;
-    ip.gl_w = dgl + pl.gl_x.size();
-    ip.nq = static_cast<int>(pl.gl_x.size());
+    ip.ft = dft;
+    ip.n_ft = static_cast<int>(pl.ft_tab.size());
+    ip.ft_inv_delta = pl.ft_inv_delta;
     ip.grid = cg;
     ip.grid_stride = G;
     ip.img = img;

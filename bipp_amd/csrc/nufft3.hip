@@ -42,14 +42,17 @@ __device__ __forceinline__ R es_kernel(R z, R beta) {
   return t > R(0) ? exp(beta * (sqrt(t) - R(1))) : R(0);
 }
 
-// 2 int_0^1 phi(z) cos(xi z) dz by Gauss-Legendre (nodes/weights on [0, 1])
-__device__ double es_ft(double xi, double beta, const double* gl_x, const double* gl_w, int nq) {
-  double s = 0;
-  for (int i = 0; i < nq; ++i) {
-    const double z = gl_x[i];
-    s += gl_w[i] * exp(beta * (sqrt(1.0 - z * z) - 1.0)) * cos(xi * z);
-  }
-  return 2.0 * s;
+// phi(xi) (the Fourier transform of the kernel) from its table by the cubic through the four
+// nearest entries; phi is even and the table starts one step below zero.
+__device__ __forceinline__ double ft_lookup(const double* tab, int n, double inv_delta,
+                                            double xi) {
+  const double u = fabs(xi) * inv_delta + 1.0;
+  int k = static_cast<int>(u);
+  k = min(max(k, 1), n - 3);
+  const double t = u - k;
+  const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
+  return tab[k - 1] * (-t * tm1 * tm2 / 6.0) + tab[k] * (tp1 * tm1 * tm2 / 2.0) +
+         tab[k + 1] * (-tp1 * t * tm2 / 2.0) + tab[k + 2] * (tp1 * t * tm1 / 6.0);
 }
 
 // --------------------------------------------------------------------------------------
@@ -763,7 +766,7 @@ __global__ __launch_bounds__(256) void n3_interp_kernel(N3InterpParams<R> p) {
       kd[o] = p.n2[d] > 1 ? (k % p.n1[d]) * p.n2[d] + k / p.n1[d] : k;
     }
     const double alpha1 = 0.5 * p.w1 * p.h1[d];
-    corr *= p.h1[d] * p.h2[d] / (alpha1 * es_ft(s * alpha1, p.beta1, p.gl_x, p.gl_w, p.nq));
+    corr *= p.h1[d] * p.h2[d] / (alpha1 * ft_lookup(p.ft, p.n_ft, p.ft_inv_delta, s * alpha1));
     cphase += p.C[d] * xq[d];
   }
   const double turns = cphase * 0.15915494309189535;

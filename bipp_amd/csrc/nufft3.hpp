@@ -117,8 +117,9 @@ struct N3InterpParams {
   double gam[3], D[3], C[3], h1[3], h2[3];
   int nf2[3], n1[3], n2[3];
   const double* lmn;             // [3][n_pix]
-  const double *gl_x, *gl_w;
-  int nq;
+  const double* ft;              // phi(xi) at xi = (j - 1) / ft_inv_delta, j < n_ft
+  int n_ft;
+  double ft_inv_delta;
   const cplx<R>* grid;
   long long grid_stride;
   void* img;                     // [L][img_ld], float or double
