@@ -835,35 +835,62 @@ __global__ void pos_range_kernel(const double* pos, long long pos_stride, int ap
 // VV_l = Z diag(w_l) Z^H for every (sample, level); Z rows = antennas used (ld ldz).
 template <typename R, typename TO>
 __global__ __launch_bounds__(256) void vv_levels_kernel(N3VVParams<R> p) {
-  __shared__ cplx<R> ti[16][17];
-  __shared__ cplx<R> tj[16][17];
+  // 64 x 64 output tile per workgroup (tiles on or below the diagonal), eigenvectors taken
+  // VK at a time through LDS; thread (tx, ty) accumulates rows i0 + tx + 16 u and columns
+  // j0 + ty + 16 v (u, v < 4), so LDS reads are consecutive over tx and broadcast over ty
+  constexpr int VT = 64, VK = 16;
+  __shared__ cplx<R> ti[VK][VT + 1];  // Z[i0 + r, e] d_e
+  __shared__ cplx<R> tj[VK][VT + 1];  // Z[j0 + c, e]
   const int sl = blockIdx.z;
   const int s = sl / p.n_levels, lv = sl % p.n_levels;
   const int n = p.n_rows ? p.n_rows[s] : p.n_rows_const;
-  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-  if (i0 >= n || j0 > i0 + 15) return;  // only blocks touching the lower triangle
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int i0 = blockIdx.x * VT, j0 = blockIdx.y * VT;
+  if (i0 >= n || j0 > i0) return;  // only tiles touching the lower triangle
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const cplx<R>* Z = p.z + (size_t)s * p.stride_z;
   const R* dw = p.wd + ((size_t)s * p.n_levels + lv) * p.n_cols;
-  cplx<R> acc = mk<R>(0, 0);
-  for (int e0 = 0; e0 < p.n_cols; e0 += 16) {
-    const int e = e0 + ty;
-    const R de = e < p.n_cols ? dw[e] : R(0);
-    // eigenvectors outside the level carry weight 0: skip chunks without any (uniform)
-    if (!__syncthreads_or(de != R(0))) continue;
-    const int i = i0 + tx, j = j0 + tx;
-    ti[ty][tx] = (e < p.n_cols && i < n) ? Z[i + (size_t)e * p.ldz] * de : mk<R>(0, 0);
-    tj[ty][tx] = (e < p.n_cols && j < n) ? Z[j + (size_t)e * p.ldz] : mk<R>(0, 0);
-    __syncthreads();
+  cplx<R> acc[4][4];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) acc = acc + mulc(ti[k][tx], tj[k][ty]);
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = mk<R>(0, 0);
+  for (int e0 = 0; e0 < p.n_cols; e0 += VK) {
+    // eigenvectors outside the level carry weight 0: skip chunks without any (uniform)
+    const int ew = e0 + (tid & (VK - 1));
+    const R dt = ew < p.n_cols ? dw[ew] : R(0);
+    if (!__syncthreads_or(dt != R(0))) continue;
+#pragma unroll
+    for (int q = 0; q < VK * VT / 256; ++q) {
+      const int idx = tid + 256 * q;
+      const int k = idx / VT, r = idx % VT, e = e0 + k;
+      const bool ok = e < p.n_cols;
+      const R de = ok ? dw[e] : R(0);
+      ti[k][r] = (ok && i0 + r < n) ? Z[(i0 + r) + (size_t)e * p.ldz] * de : mk<R>(0, 0);
+      tj[k][r] = (ok && j0 + r < n) ? Z[(j0 + r) + (size_t)e * p.ldz] : mk<R>(0, 0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < VK; ++k) {
+      cplx<R> a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = ti[k][tx + 16 * u];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) b[v] = tj[k][ty + 16 * v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = acc[u][v] + mulc(a[u], b[v]);
+    }
     __syncthreads();
   }
-  const int i = i0 + tx, j = j0 + ty;
-  if (i < n && j < n && i >= j) {
-    cplx<TO>* out = reinterpret_cast<cplx<TO>*>(p.vv) + (size_t)sl * p.vv_stride;
-    out[i + (size_t)p.ld_vv * j] = cvt<TO>(acc);
-  }
+  cplx<TO>* out = reinterpret_cast<cplx<TO>*>(p.vv) + (size_t)sl * p.vv_stride;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + tx + 16 * u, j = j0 + ty + 16 * v;
+      if (i < n && j < n && i >= j) out[i + (size_t)p.ld_vv * j] = cvt<TO>(acc[u][v]);
+    }
 }
 
 // Trace of VV_l for every (sample, level) into the unused upper-triangle slot (0, 1).
@@ -1078,7 +1105,7 @@ template void launch_vv_trace<double>(int, int, const int*, int, void*, long lon
 template <typename R, typename TO>
 void launch_vv_levels(const N3VVParams<R>& p, int n_samples, int n_rows_max, hipStream_t st) {
   if (n_samples <= 0 || n_rows_max <= 0) return;
-  dim3 grid(ceil_div(n_rows_max, 16), ceil_div(n_rows_max, 16), n_samples * p.n_levels);
+  dim3 grid(ceil_div(n_rows_max, 64), ceil_div(n_rows_max, 64), n_samples * p.n_levels);
   vv_levels_kernel<R, TO><<<grid, 256, 0, st>>>(p);
   check_launch("vv_levels_kernel");
 }
