@@ -120,12 +120,26 @@ struct ContextImpl {
     BIPP_HIP(hipGetDeviceProperties(&prop, device));
     n_cu = prop.multiProcessorCount;
     BIPP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    BIPP_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     pu = BIPP_PU_GPU;
   }
   ~ContextImpl() {
+    if (ready) (void)hipEventDestroy(ready);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  void sync() { BIPP_HIP(hipStreamSynchronize(stream)); }
+  // Waits for the stream by polling an event: a blocking wait that spans a long kernel wakes
+  // up about a millisecond late, which leaves the device idle between batches.
+  void sync() {
+    BIPP_HIP(hipEventRecord(ready, stream));
+    for (;;) {
+      const hipError_t q = hipEventQuery(ready);
+      if (q == hipSuccess) break;
+      if (q != hipErrorNotReady) BIPP_HIP(q);
+    }
+  }
+
+ private:
+  hipEvent_t ready = nullptr;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -232,7 +246,7 @@ struct EigRunner {
     // persistent launch, largest problems first
     std::vector<int> h_nred(nS);
     BIPP_HIP(hipMemcpyAsync(h_nred.data(), fp.n_red, sizeof(int) * nS, hipMemcpyDeviceToHost, st));
-    BIPP_HIP(hipStreamSynchronize(st));
+    ctx.sync();
     std::vector<int> h_order(nS);
     std::iota(h_order.begin(), h_order.end(), 0);
     std::stable_sort(h_order.begin(), h_order.end(),
@@ -341,7 +355,7 @@ struct EigRunner {
     h_info.resize(n_samples);
     BIPP_HIP(hipMemcpyAsync(h_info.data(), info.p, sizeof(int) * n_samples, hipMemcpyDeviceToHost,
                             ctx.stream));
-    BIPP_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.sync();
     for (int k = 0; k < n_samples; ++k)
       if (h_info[k] != 0)
         fail(BIPP_EIGENSOLVER_ERROR, "generalized eigensolve failed for sample " +
@@ -986,6 +1000,8 @@ struct BippSynthesisImpl {
   // NUFFT sub-stages: 4 virtual vis, 5 point sort, 6 spread, 7 deconvolve + FFT, 8 interpolate
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_n3[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* h_rg = nullptr;         // pinned per-sample position ranges
+  size_t h_rg_cap = 0;
   static constexpr int N_TIMES = 9;
   double stage_ms[N_TIMES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool n3_timed = false;
@@ -995,6 +1011,7 @@ struct BippSynthesisImpl {
     for (auto& e : ev_n3) BIPP_HIP(hipEventCreate(&e));
   }
   ~BippSynthesisImpl() {
+    if (h_rg) (void)hipHostFree(h_rg);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : ev_n3)
@@ -1151,9 +1168,16 @@ struct BippSynthesisImpl {
       // input-point extent from the per-sample position ranges
       double* rg = ranges.as<double>(6 * (size_t)nS);
       launch_pos_range(nS, pp.pos, pp.pos_stride, apad, pp.n_ant, 0, rg, st);
-      std::vector<double> h_rg(6 * (size_t)nS);
-      BIPP_HIP(hipMemcpyAsync(h_rg.data(), rg, sizeof(double) * h_rg.size(), hipMemcpyDeviceToHost, st));
-      BIPP_HIP(hipStreamSynchronize(st));
+      // pinned host copy: the copy is asynchronous and the host only polls the event below
+      const size_t n_rg = 6 * (size_t)nS;
+      if (h_rg_cap < n_rg) {
+        if (h_rg) BIPP_HIP(hipHostFree(h_rg));
+        h_rg = nullptr;
+        BIPP_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_rg), sizeof(double) * n_rg));
+        h_rg_cap = n_rg;
+      }
+      BIPP_HIP(hipMemcpyAsync(h_rg, rg, sizeof(double) * n_rg, hipMemcpyDeviceToHost, st));
+      c.sync();  // the ranges arrive after the eigensolve
       double X[3] = {0, 0, 0}, Cc[3] = {0, 0, 0}, S[3], D[3];
       for (int k = 0; k < nS; ++k)
         for (int d = 0; d < 3; ++d)
@@ -1241,7 +1265,7 @@ struct BippSynthesisImpl {
     std::vector<R> tmp((size_t)nS * n_beam);
     BIPP_HIP(hipMemcpyAsync(tmp.data(), er.d_d(), sizeof(R) * tmp.size(), hipMemcpyDeviceToHost, st));
     BIPP_HIP(hipMemcpyAsync(last_nred.data(), er.d_n_red(), sizeof(int) * nS, hipMemcpyDeviceToHost, st));
-    BIPP_HIP(hipStreamSynchronize(st));
+    c.sync();
     for (size_t i = 0; i < tmp.size(); ++i) last_d[i] = static_cast<double>(tmp[i]);
     n_collected += nS;
   }

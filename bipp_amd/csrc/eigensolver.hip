@@ -253,6 +253,39 @@ __device__ void gemm_tiles(int n, FA fa, FB fb, KR krange, cplx<R>* __restrict__
   }
 }
 
+// Sum over the 64 lanes of a wavefront (every lane gets it; DPP within rows of 16, then
+// two lane swaps).
+template <typename R>
+__device__ __forceinline__ R wave_total(R v) {
+  v = sum16(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// Sums of a and b over the workgroup (every thread gets them), with one barrier: the
+// caller guarantees that another barrier separates this call from the next write of
+// sh.red (it holds 2 values per wavefront).
+template <typename R, int NMAX>
+__device__ __forceinline__ void block_sum2(R& a, R& b, EigShared<R, NMAX>& sh) {
+  constexpr int NW_ = EigCfg<NMAX>::NW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  a = wave_total(a);
+  b = wave_total(b);
+  if (lane == 0) {
+    sh.red[wave] = a;
+    sh.red[NW_ + wave] = b;
+  }
+  __syncthreads();
+  a = 0;
+  b = 0;
+#pragma unroll
+  for (int w = 0; w < NW_; ++w) {
+    a += sh.red[w];
+    b += sh.red[NW_ + w];
+  }
+}
+
 // Householder reflector (zlarfg semantics) annihilating A[c+2:n, c]; writes v (v[c+1]=1)
 // into vec and into A[c+2:n, c], beta into A[c+1, c] and ee[c]; returns tau.
 template <typename R, int NMAX>
@@ -262,9 +295,10 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
   const int tid = threadIdx.x;
   cplx<R>* col = A + (size_t)c * ld;
   const cplx<R> alpha = col[c + 1];
-  R part = 0;
+  R part = 0, unused = 0;
   for (int i = c + 2 + tid; i < n; i += NT_) part += abs2(col[i]);
-  const R xnorm2 = block_sum<NT_>(part, sh.red);
+  block_sum2(part, unused, sh);  // sh.red is next written after this function's barrier
+  const R xnorm2 = part;
   cplx<R> tau, scal;
   R beta;
   if (xnorm2 == R(0) && alpha.im == R(0)) {
@@ -289,16 +323,6 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
   }
   __syncthreads();
   return tau;
-}
-
-// Sum over the 64 lanes of a wavefront (every lane gets it; DPP within rows of 16, then
-// two lane swaps).
-template <typename R>
-__device__ __forceinline__ R wave_total(R v) {
-  v = sum16(v);
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
 }
 
 // One pass over the lower triangle of the trailing square [i0,n)^2 of the Hermitian A
@@ -408,9 +432,8 @@ __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<
     pr += t.re;
     pi += t.im;
   }
-  const R dr = block_sum<NT_>(pr, sh.red);
-  const R di = block_sum<NT_>(pi, sh.red);
-  const cplx<R> alpha2 = tau * mk<R>(R(-0.5) * dr, R(-0.5) * di);
+  block_sum2(pr, pi, sh);  // sh.red is next written after this function's barrier
+  const cplx<R> alpha2 = tau * mk<R>(R(-0.5) * pr, R(-0.5) * pi);
   for (int i = c + 1 + tid; i < n; i += NT_) sh.w[i] = sh.w[i] + alpha2 * vec[i];
   __syncthreads();
 }
