@@ -2,16 +2,18 @@
 //
 // Restates, per sample, src/eigensolver.cpp:129-225 (flag -> reduce -> Gram -> ?hegv)
 // with the LAPACK steps of ?hegv(itype=1,'V','L') (src/host/lapack_api.hpp:112-158):
-//   Cholesky G = L L^H, C = L^-1 S L^-H, Householder tridiagonalisation C = Q T Q^H,
-//   implicit QL on T (eigenvalues + rotations), X = Q Z, eigenvectors L^-H X,
-//   eigenvalues ascending.
+//   Cholesky G = L L^H (left-looking), W = L^-1, C = W S W^H (two LDS-tiled GEMMs),
+//   Householder tridiagonalisation C = Q T Q^H on the lower triangle, implicit QL on T
+//   (eigenvalues + rotations into Z, sweeps generated and applied in pairs), X = Q Z
+//   (eigenvector columns in registers), eigenvectors W^H X (GEMM), eigenvalues ascending.
 //
-// Launch shape: a persistent grid of 256-thread workgroups; each workgroup pulls sample
-// indices from a device counter (largest problems first) and solves one problem at a
-// time, keeping its matrices in a per-workgroup slot of global memory (L2/MALL resident
-// at these sizes) and its vectors in LDS.  Every step is a workgroup-wide parallel
-// update separated by barriers, with lanes mapped to matrix rows so that global
-// accesses are coalesced.
+// Launch shape: a persistent grid (512-thread workgroups for problems above 128, 256
+// below); each workgroup pulls sample indices from a device counter (largest problems
+// first) and solves one problem at a time, keeping its matrices in a per-workgroup slot of
+// global memory and its vectors, tiles and rotation buffers in LDS.  The column steps are
+// workgroup-wide parallel updates separated by barriers, with lanes mapped to matrix rows
+// so that global accesses are coalesced; the QL rotation chain runs on one lane of a
+// dedicated wavefront.
 #include "common.hpp"
 #include "kernels.hpp"
 
