@@ -743,21 +743,25 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
 // --------------------------------------------------------------------------------------
 // Interpolation at the pixels, correction, real part accumulated into the images.
 // --------------------------------------------------------------------------------------
-template <typename R, typename TI>
+// The kernel width is a template argument so that the per-axis weights and indices stay in
+// registers (a run-time width puts them in scratch memory, read in the innermost loop).
+template <typename R, typename TI, int W>
 __global__ __launch_bounds__(256) void n3_interp_kernel(N3InterpParams<R> p) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= p.n_pix) return;
-  const int w = p.w2;
-  R wx[16], wy[16], wz[16];
-  int kx[16], ky[16], kz[16];
+  constexpr int w = W;
+  R wx[W], wy[W], wz[W];
+  int kx[W], ky[W], kz[W];
   double corr = 1.0, cphase = 0.0;
   const double xq[3] = {p.lmn[q], p.lmn[p.n_pix + q], p.lmn[2 * (size_t)p.n_pix + q]};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double s = p.gam[d] * (xq[d] - p.D[d]);
     const double a2 = p.h1[d] * s / p.h2[d];
     const int k0 = static_cast<int>(ceil(a2 - 0.5 * w));
     R* wd = d == 0 ? wx : (d == 1 ? wy : wz);
     int* kd = d == 0 ? kx : (d == 1 ? ky : kz);
+#pragma unroll
     for (int o = 0; o < w; ++o) {
       wd[o] = es_kernel<R>(static_cast<R>((k0 + o - a2) * 2.0 / w), R(p.beta2));
       int k = (k0 + o) % p.nf2[d];
@@ -775,11 +779,14 @@ __global__ __launch_bounds__(256) void n3_interp_kernel(N3InterpParams<R> p) {
   for (int lv = 0; lv < p.n_levels; ++lv) {
     const cplx<R>* g = p.grid + (size_t)lv * p.grid_stride;
     R sr = 0, si = 0;
+#pragma unroll 1
     for (int oz = 0; oz < w; ++oz) {
+#pragma unroll 1
       for (int oy = 0; oy < w; ++oy) {
         const cplx<R>* row = g + ((size_t)kz[oz] * p.nf2[1] + ky[oy]) * p.nf2[0];
         const R wyz = wy[oy] * wz[oz];
         R rr = 0, ri = 0;
+#pragma unroll
         for (int ox = 0; ox < w; ++ox) {
           const cplx<R> v = row[kx[ox]];
           rr += wx[ox] * v.re;
@@ -1081,8 +1088,16 @@ template void launch_fft_pass<double>(const N3PassParams<double>&, hipStream_t);
 template <typename R, typename TI>
 void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st) {
   if (p.n_pix <= 0) return;
-  if (p.w2 > 16) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT kernel width above 16");
-  n3_interp_kernel<R, TI><<<ceil_div(p.n_pix, 256), 256, 0, st>>>(p);
+  const dim3 grid(ceil_div(p.n_pix, 256)), block(256);
+  switch (p.w2) {
+#define BIPP_INTERP_W(W) \
+  case W: n3_interp_kernel<R, TI, W><<<grid, block, 0, st>>>(p); break;
+    BIPP_INTERP_W(2) BIPP_INTERP_W(3) BIPP_INTERP_W(4) BIPP_INTERP_W(5) BIPP_INTERP_W(6)
+    BIPP_INTERP_W(7) BIPP_INTERP_W(8) BIPP_INTERP_W(9) BIPP_INTERP_W(10) BIPP_INTERP_W(11)
+    BIPP_INTERP_W(12) BIPP_INTERP_W(13) BIPP_INTERP_W(14) BIPP_INTERP_W(15) BIPP_INTERP_W(16)
+#undef BIPP_INTERP_W
+    default: fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT kernel width outside [2, 16]");
+  }
   check_launch("n3_interp_kernel");
 }
 template void launch_n3_interp<float, float>(const N3InterpParams<float>&, hipStream_t);
