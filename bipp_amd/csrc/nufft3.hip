@@ -41,6 +41,17 @@ __device__ __forceinline__ R es_kernel(R z, R beta) {
   const R t = R(1) - z * z;
   return t > R(0) ? exp(beta * (sqrt(t) - R(1))) : R(0);
 }
+// float: the exponent beta (sqrt(t) - 1) lies in [-beta, 0] (beta <= 2.38 * 16), so the
+// hardware base-2 exponential and square root (about 1 ulp each, normal arguments only) need
+// no range reduction or denormal scaling; their error is below the ~1e-6 relative error that
+// rounding the exponent itself already gives.
+template <>
+__device__ __forceinline__ float es_kernel<float>(float z, float beta) {
+  const float t = 1.0f - z * z;
+  const float e = __builtin_amdgcn_exp2f(beta * 1.44269504088896341f *
+                                         (__builtin_amdgcn_sqrtf(fmaxf(t, 0.0f)) - 1.0f));
+  return t > 0.0f ? e : 0.0f;
+}
 
 // phi(xi) (the Fourier transform of the kernel) from its table by the cubic through the four
 // nearest entries; phi is even and the table starts one step below zero.
