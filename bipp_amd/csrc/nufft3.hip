@@ -181,10 +181,9 @@ __global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, c
 // --------------------------------------------------------------------------------------
 template <typename R>
 struct SpreadBatch {
-  int porg[64][2];        // y, z of l0 - block origin
-  R pwx[64][SP_BIN];      // x weight of each block column
-  R pwy[64][SP_WMAX];     // kernel weights along y, z by kernel offset
-  R pwz[64][SP_WMAX];
+  R pwx[64][SP_BIN];      // kernel weight of each block column (x), row (y) and slab (z),
+  R pwy[64][SP_BIN];      // zero outside the point's support
+  R pwz[64][SP_BIN];
   R pv[64][SP_LG][2];     // values of the levels of this launch
 };
 
@@ -200,7 +199,7 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
   // per-wavefront batch buffers; reused at the end for the reduction of the row sums
   __shared__ union {
     SpreadBatch<R> batch[NWV];
-    R red[NWV - 1][64][B * SP_LG * 2];
+    R red[NWV - 1][64][(B / 2) * SP_LG * 2];  // half of a lane's cells at a time
   } sh;
   const int w = p.w;
   const int L = p.n_levels;
@@ -270,8 +269,6 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     const int slot = __popcll(mask & ((1ull << lane) - 1ull));
     wave_sync();  // the previous batch of this wavefront fully consumed
     if (hit) {
-      sb.porg[slot][0] = rel[1];
-      sb.porg[slot][1] = rel[2];
 #pragma unroll
       for (int cx = 0; cx < B; ++cx) {
         const int o = cx - rel[0];
@@ -279,9 +276,15 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
                                ? es_kernel<R>((R(o) - R(pt.fr[0])) * two_over_w, beta)
                                : R(0);
       }
-      for (int o = 0; o < w; ++o) {
-        sb.pwy[slot][o] = es_kernel<R>((R(o) - R(pt.fr[1])) * two_over_w, beta);
-        sb.pwz[slot][o] = es_kernel<R>((R(o) - R(pt.fr[2])) * two_over_w, beta);
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        const int oy = c - rel[1], oz = c - rel[2];
+        sb.pwy[slot][c] = (oy >= 0 && oy < w)
+                              ? es_kernel<R>((R(oy) - R(pt.fr[1])) * two_over_w, beta)
+                              : R(0);
+        sb.pwz[slot][c] = (oz >= 0 && oz < w)
+                              ? es_kernel<R>((R(oz) - R(pt.fr[2])) * two_over_w, beta)
+                              : R(0);
       }
       const R m = R(pt.mult);
       const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
@@ -298,15 +301,9 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     }
     wave_sync();
     n_hit += total;
-    // Branch-free: rows outside a point's support take weight 0 (the weight reads use
-    // clamped, in-range offsets); two points per iteration keep two load chains in flight.
-    auto row_weight = [&](int q) {
-      const int ly = cy - sb.porg[q][0], lz = cz - sb.porg[q][1];
-      const bool in = (unsigned)ly < (unsigned)w && (unsigned)lz < (unsigned)w;
-      const int lyc = min(max(ly, 0), w - 1), lzc = min(max(lz, 0), w - 1);
-      const R wyz = sb.pwy[q][lyc] * sb.pwz[q][lzc];
-      return in ? wyz : R(0);
-    };
+    // Branch-free: rows outside a point's support take weight 0 (the expanded y and z
+    // weights are zero there); two points per iteration keep two load chains in flight.
+    auto row_weight = [&](int q) { return sb.pwy[q][cy] * sb.pwz[q][cz]; };
     auto add_point = [&](int q, R wyz) {
       R vr[SP_LG], vi[SP_LG];
 #pragma unroll
@@ -332,29 +329,34 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     }
     if (q < total) add_point(q, row_weight(q));
   }
-  // ---- reduce the wavefronts' row sums into wavefront 0 --------------------------------
+  // ---- reduce the wavefronts' row sums into wavefront 0, half a row at a time ----------
   if (n_active > 1) {
-    __syncthreads();  // every wavefront is done with its batch buffers
-    if (wave > 0 && wave < n_active) {
-      R* dst = sh.red[wave - 1][lane];
 #pragma unroll
-      for (int c = 0; c < B; ++c)
+    for (int h = 0; h < 2; ++h) {
+      // every wavefront is done with its batch buffers (h = 0) or wavefront 0 with the
+      // previous half (h = 1)
+      __syncthreads();
+      if (wave > 0 && wave < n_active) {
+        R* dst = sh.red[wave - 1][lane];
 #pragma unroll
-        for (int lv = 0; lv < SP_LG; ++lv) {
-          dst[(c * SP_LG + lv) * 2 + 0] = acc[c][lv][0];
-          dst[(c * SP_LG + lv) * 2 + 1] = acc[c][lv][1];
-        }
-    }
-    __syncthreads();
-    for (int k = 0; k < n_active - 1 && wave == 0; ++k) {
-      const R* src = sh.red[k][lane];
+        for (int c = 0; c < B / 2; ++c)
 #pragma unroll
-      for (int c = 0; c < B; ++c)
+          for (int lv = 0; lv < SP_LG; ++lv) {
+            dst[(c * SP_LG + lv) * 2 + 0] = acc[h * (B / 2) + c][lv][0];
+            dst[(c * SP_LG + lv) * 2 + 1] = acc[h * (B / 2) + c][lv][1];
+          }
+      }
+      __syncthreads();
+      for (int k = 0; k < n_active - 1 && wave == 0; ++k) {
+        const R* src = sh.red[k][lane];
 #pragma unroll
-        for (int lv = 0; lv < SP_LG; ++lv) {
-          acc[c][lv][0] += src[(c * SP_LG + lv) * 2 + 0];
-          acc[c][lv][1] += src[(c * SP_LG + lv) * 2 + 1];
-        }
+        for (int c = 0; c < B / 2; ++c)
+#pragma unroll
+          for (int lv = 0; lv < SP_LG; ++lv) {
+            acc[h * (B / 2) + c][lv][0] += src[(c * SP_LG + lv) * 2 + 0];
+            acc[h * (B / 2) + c][lv][1] += src[(c * SP_LG + lv) * 2 + 1];
+          }
+      }
     }
   }
   if (wave > 0) return;  // no barrier below: wavefront 0 writes the block
