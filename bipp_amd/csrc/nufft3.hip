@@ -186,6 +186,35 @@ struct SpreadBatch {
   R pwz[64][SP_BIN];
   R pv[64][SP_LG][2];     // values of the levels of this launch
 };
+static_assert(SP_BIN == 8 && SP_LG * 2 == 8, "batch rows are moved as 8 values");
+
+// Rows of 8 values moved with 16-byte LDS accesses: a wavefront's row stores (rows 32 or 64
+// bytes apart) conflict at most 2-way instead of 8-way for 4-byte stores, and a row read
+// is two or four instructions.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_row8(float* dst, const float (&v)[8]) {
+  reinterpret_cast<f32x4*>(dst)[0] = f32x4{v[0], v[1], v[2], v[3]};
+  reinterpret_cast<f32x4*>(dst)[1] = f32x4{v[4], v[5], v[6], v[7]};
+}
+__device__ __forceinline__ void st_row8(double* dst, const double (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) reinterpret_cast<f64x2*>(dst)[k] = f64x2{v[2 * k], v[2 * k + 1]};
+}
+__device__ __forceinline__ void ld_row8(const float* src, float (&v)[8]) {
+  const f32x4 a = reinterpret_cast<const f32x4*>(src)[0];
+  const f32x4 b = reinterpret_cast<const f32x4*>(src)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void ld_row8(const double* src, double (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const f64x2 a = reinterpret_cast<const f64x2*>(src)[k];
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
 
 // wavefronts per spread block (batches split between them); static LDS stays under 64 KiB
 template <typename R>
@@ -197,7 +226,7 @@ template <typename R, typename TV>
 __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
   constexpr int B = SP_BIN, PB = 64, NWV = SpreadWaves<R>::value;
   // per-wavefront batch buffers; reused at the end for the reduction of the row sums
-  __shared__ union {
+  __shared__ __align__(16) union {
     SpreadBatch<R> batch[NWV];
     R red[NWV - 1][64][(B / 2) * SP_LG * 2];  // half of a lane's cells at a time
   } sh;
@@ -269,35 +298,33 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     const int slot = __popcll(mask & ((1ull << lane) - 1ull));
     wave_sync();  // the previous batch of this wavefront fully consumed
     if (hit) {
+      R row[3][B];
 #pragma unroll
-      for (int cx = 0; cx < B; ++cx) {
-        const int o = cx - rel[0];
-        sb.pwx[slot][cx] = (o >= 0 && o < w)
-                               ? es_kernel<R>((R(o) - R(pt.fr[0])) * two_over_w, beta)
-                               : R(0);
-      }
+      for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int c = 0; c < B; ++c) {
-        const int oy = c - rel[1], oz = c - rel[2];
-        sb.pwy[slot][c] = (oy >= 0 && oy < w)
-                              ? es_kernel<R>((R(oy) - R(pt.fr[1])) * two_over_w, beta)
-                              : R(0);
-        sb.pwz[slot][c] = (oz >= 0 && oz < w)
-                              ? es_kernel<R>((R(oz) - R(pt.fr[2])) * two_over_w, beta)
-                              : R(0);
-      }
+        for (int c = 0; c < B; ++c) {
+          const int o = c - rel[d];
+          row[d][c] = (o >= 0 && o < w)
+                          ? es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta)
+                          : R(0);
+        }
+      st_row8(sb.pwx[slot], row[0]);
+      st_row8(sb.pwy[slot], row[1]);
+      st_row8(sb.pwz[slot], row[2]);
       const R m = R(pt.mult);
       const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
+      R pvr[SP_LG * 2];
 #pragma unroll
       for (int lv = 0; lv < SP_LG; ++lv) {
         if (lv < L) {
           const TV c = vals[lv];
-          sb.pv[slot][lv][0] = m * (R(c.re) * pr - R(c.im) * pim);
-          sb.pv[slot][lv][1] = m * (R(c.re) * pim + R(c.im) * pr);
+          pvr[2 * lv] = m * (R(c.re) * pr - R(c.im) * pim);
+          pvr[2 * lv + 1] = m * (R(c.re) * pim + R(c.im) * pr);
         } else {
-          sb.pv[slot][lv][0] = sb.pv[slot][lv][1] = R(0);
+          pvr[2 * lv] = pvr[2 * lv + 1] = R(0);
         }
       }
+      st_row8(&sb.pv[slot][0][0], pvr);
     }
     wave_sync();
     n_hit += total;
@@ -305,15 +332,17 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     // weights are zero there); two points per iteration keep two load chains in flight.
     auto row_weight = [&](int q) { return sb.pwy[q][cy] * sb.pwz[q][cz]; };
     auto add_point = [&](int q, R wyz) {
-      R vr[SP_LG], vi[SP_LG];
+      R vr[SP_LG], vi[SP_LG], pvq[SP_LG * 2], wxq[B];
+      ld_row8(&sb.pv[q][0][0], pvq);
+      ld_row8(sb.pwx[q], wxq);
 #pragma unroll
       for (int lv = 0; lv < SP_LG; ++lv) {
-        vr[lv] = sb.pv[q][lv][0] * wyz;
-        vi[lv] = sb.pv[q][lv][1] * wyz;
+        vr[lv] = pvq[2 * lv] * wyz;
+        vi[lv] = pvq[2 * lv + 1] * wyz;
       }
 #pragma unroll
       for (int cx = 0; cx < B; ++cx) {
-        const R wx = sb.pwx[q][cx];
+        const R wx = wxq[cx];
 #pragma unroll
         for (int lv = 0; lv < SP_LG; ++lv) {
           acc[cx][lv][0] += wx * vr[lv];
