@@ -470,8 +470,18 @@ int fft_pass_radix_max() {
   return v;
 }
 
+// multiplier and shift for exact division by d (see N3Radix)
+void div_magic(int d, unsigned& m, int& s) {
+  if (d < 1) fail(BIPP_INTERNAL_ERROR, "FFT divisor below 1");
+  s = 0;
+  while ((1LL << s) < d) ++s;
+  m = static_cast<unsigned>((((1ULL << 32) * ((1ULL << s) - static_cast<unsigned long long>(d))) /
+                             static_cast<unsigned long long>(d)) + 1ULL);
+}
+
 N3Radix radices_of(int m, int max_radix) {
   N3Radix r;
+  const int n = m;
   int twos = 0, threes = 0;
   while (m % 2 == 0) {
     ++twos;
@@ -502,6 +512,13 @@ N3Radix radices_of(int m, int max_radix) {
   }
   if (m != 1) fail(BIPP_INTERNAL_ERROR, "FFT length is not 2^a 3^b 5^c");
   std::sort(r.r, r.r + r.count);
+  div_magic(n, r.n_m, r.n_s);
+  int ns = 1;
+  for (int st = 0; st < r.count; ++st) {
+    div_magic(n / r.r[st], r.nb_m[st], r.nb_s[st]);
+    div_magic(ns, r.ns_m[st], r.ns_s[st]);
+    ns *= r.r[st];
+  }
   return r;
 }
 

@@ -561,17 +561,24 @@ __device__ __forceinline__ Twid<R> load_twid(const cplx<R>* __restrict__ g, int 
   return Twid<R>{lds, lds + 64};
 }
 
-// One Stockham stage of radix RAD over nl lines (stride ld) of length n.
+// x / d for 0 <= x < 2^31 from the multiplier and shift of d (N3Radix)
+__device__ __forceinline__ int fast_div(int x, unsigned m, int s) {
+  return static_cast<int>((__umulhi(static_cast<unsigned>(x), m) + static_cast<unsigned>(x)) >> s);
+}
+
+// One Stockham stage of radix RAD over nl lines (stride ld) of length n; (nb_m, nb_s) and
+// (ns_m, ns_s) divide by n / RAD and by ns.
 template <typename R, int RAD>
 __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n, int nl, int ld,
-                                          int ns, const Twid<R>& tw, int tid, int nt) {
+                                          int ns, const Twid<R>& tw, int tid, int nt,
+                                          unsigned nb_m, int nb_s, unsigned ns_m, int ns_s) {
   const int nb = n / RAD;
   const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
   for (int t = tid; t < nl * nb; t += nt) {
-    const int line = t / nb, j = t - line * nb;
+    const int line = fast_div(t, nb_m, nb_s), j = t - line * nb;
     const cplx<R>* src = in + (size_t)line * ld;
     cplx<R>* dst = out + (size_t)line * ld;
-    const int jq = j / ns;
+    const int jq = fast_div(j, ns_m, ns_s);
     const int k = j - jq * ns;
     // twiddles w^r, w = exp(2 pi i k tstride / n): w, w^4 and w^8 from the table, the
     // others as products at most two deep (k r tstride < n for r < RAD)
@@ -613,24 +620,26 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
   cplx<R>* out = b;
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
+#define DIVS rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st]
     switch (R_) {
-      case 2: fft_stage<R, 2>(in, out, n, nl, ld, ns, tw, tid, nt); break;
-      case 3: fft_stage<R, 3>(in, out, n, nl, ld, ns, tw, tid, nt); break;
-      case 4: fft_stage<R, 4>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      case 2: fft_stage<R, 2>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+      case 3: fft_stage<R, 3>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+      case 4: fft_stage<R, 4>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
       case 6:
-        if constexpr (MAXR >= 6) fft_stage<R, 6>(in, out, n, nl, ld, ns, tw, tid, nt);
+        if constexpr (MAXR >= 6) fft_stage<R, 6>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
         break;
       case 8:
-        if constexpr (MAXR >= 8) fft_stage<R, 8>(in, out, n, nl, ld, ns, tw, tid, nt);
+        if constexpr (MAXR >= 8) fft_stage<R, 8>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
         break;
       case 9:
-        if constexpr (MAXR >= 9) fft_stage<R, 9>(in, out, n, nl, ld, ns, tw, tid, nt);
+        if constexpr (MAXR >= 9) fft_stage<R, 9>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
         break;
       case 16:
-        if constexpr (MAXR >= 16) fft_stage<R, 16>(in, out, n, nl, ld, ns, tw, tid, nt);
+        if constexpr (MAXR >= 16) fft_stage<R, 16>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
         break;
-      default: fft_stage<R, 5>(in, out, n, nl, ld, ns, tw, tid, nt); break;
+      default: fft_stage<R, 5>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
     }
+#undef DIVS
     __syncthreads();
     ns *= R_;
     cplx<R>* tmp = in;
@@ -685,7 +694,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
     for (int k = 0; k < FFT_PF; ++k) {
       const int i = tid + k * FFT_NT;
       if (i < nlg * n) {
-        const int li = i / n, x = i - li * n;
+        const int li = fast_div(i, p.rad.n_m, p.rad.n_s), x = i - li * n;
         regs[k] = in_window(x, p.in, n) ? p.grid[lbase[buf][li] + x] : mk<R>(0, 0);
       }
     }
@@ -710,7 +719,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
     if (gn < n_groups) fetch(gn, cb ^ 1, nxt);  // in flight during the transform
     cplx<R>* res = lds_fft<R, MAXR>(a, b, n, nlg, n, p.rad, tw, tid, FFT_NT);
     for (int i = tid; i < nlg * n; i += FFT_NT) {
-      const int li = i / n, k = i - li * n;
+      const int li = fast_div(i, p.rad.n_m, p.rad.n_s), k = i - li * n;
       if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
     }
     __syncthreads();

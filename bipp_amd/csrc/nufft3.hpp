@@ -8,6 +8,13 @@ namespace bippamd {
 struct N3Radix {
   int count = 0;
   int r[20] = {};
+  // Division by run-time constants as q = (umulhi(x, m) + x) >> s, exact for 0 <= x < 2^31
+  // (s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1): the length n, and per stage the
+  // butterfly count n / r and the span ns = r[0] ... r[st-1].
+  unsigned n_m = 1;
+  int n_s = 0;
+  unsigned nb_m[20] = {}, ns_m[20] = {};
+  int nb_s[20] = {}, ns_s[20] = {};
 };
 
 // One non-uniform point of the spread (one antenna pair a >= b of one sample).
