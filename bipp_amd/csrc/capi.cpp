@@ -473,10 +473,7 @@ int fft_pass_radix_max() {
 // multiplier and shift for exact division by d (see N3Radix)
 void div_magic(int d, unsigned& m, int& s) {
   if (d < 1) fail(BIPP_INTERNAL_ERROR, "FFT divisor below 1");
-  s = 0;
-  while ((1LL << s) < d) ++s;
-  m = static_cast<unsigned>((((1ULL << 32) * ((1ULL << s) - static_cast<unsigned long long>(d))) /
-                             static_cast<unsigned long long>(d)) + 1ULL);
+  n3_div_magic(d, m, s);
 }
 
 N3Radix radices_of(int m, int max_radix) {

@@ -750,15 +750,18 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
   // the launcher checks that the item count fits in 31 bits
   const int n_items = n_xt * p.n_sub * p.other.count * p.n_levels;
   for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const int xt = item % n_xt;
-    int rest = item / n_xt;
-    const int sub = rest % p.n_sub;
-    rest /= p.n_sub;
-    const int o = rest % p.other.count;
-    const int lv = rest / p.other.count;
+    const int q1 = fast_div(item, p.xt_m, p.xt_s);
+    const int xt = item - q1 * n_xt;
+    const int q2 = fast_div(q1, p.sub_m, p.sub_s);
+    const int sub = q1 - q2 * p.n_sub;
+    const int lv = fast_div(q2, p.oc_m, p.oc_s);
+    const int o = q2 - lv * p.other.count;
     int oo = p.other.start + o;
     oo -= oo >= p.no ? p.no : 0;
-    if (p.operm_n2 > 1) oo = (oo % p.operm_n1) * p.operm_n2 + oo / p.operm_n1;
+    if (p.operm_n2 > 1) {
+      const int q = fast_div(oo, p.op_m, p.op_s);
+      oo = (oo - q * p.operm_n1) * p.operm_n2 + q;
+    }
     cplx<R>* base = p.grid + (size_t)lv * p.grid_stride + (size_t)oo * p.outer_stride;
     for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
       const int xi = t & (TW - 1), i = t >> tw_shift;
@@ -1099,7 +1102,8 @@ template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
 template void launch_fft_lines<double>(const N3LineParams<double>&, hipStream_t);
 
 template <typename R>
-void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
+void launch_fft_pass(const N3PassParams<R>& p_in, hipStream_t st) {
+  N3PassParams<R> p = p_in;
   if (!window_ok(p.xw, p.nx) || !window_ok(p.other, p.no) || !window_ok(p.in, p.nax) ||
       !window_ok(p.out, p.nax) || (long long)p.m * p.n_sub != p.nax ||
       (p.twiddle && p.ntot != p.nax))
@@ -1114,6 +1118,11 @@ void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st) {
       (long long)((p.xw.count + TW - 1) / TW) * p.n_sub * p.other.count * p.n_levels;
   if (items <= 0) return;
   if (items > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT grid too large for one pass");
+  if (p.operm_n2 > 1 && p.operm_n1 < 1) fail(BIPP_INTERNAL_ERROR, "FFT pass permutation");
+  n3_div_magic((p.xw.count + TW - 1) / TW, p.xt_m, p.xt_s);
+  n3_div_magic(p.n_sub, p.sub_m, p.sub_s);
+  n3_div_magic(p.other.count, p.oc_m, p.oc_s);
+  n3_div_magic(p.operm_n2 > 1 ? p.operm_n1 : 1, p.op_m, p.op_s);
   const size_t sh = sizeof(cplx<R>) * (2 * (size_t)TW * ld + twid_len(p.m) +
                                        (p.twiddle ? twid_len(static_cast<int>(p.ntot)) : 0));
   if (sh > 160 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");

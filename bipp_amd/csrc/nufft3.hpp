@@ -5,6 +5,15 @@
 
 namespace bippamd {
 
+// Multiplier m and shift s with x / d = (umulhi(x, m) + x) >> s for 0 <= x < 2^31, d >= 1:
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+inline void n3_div_magic(int d, unsigned& m, int& s) {
+  s = 0;
+  while ((1LL << s) < d) ++s;
+  m = static_cast<unsigned>((((1ULL << 32) * ((1ULL << s) - static_cast<unsigned long long>(d))) /
+                             static_cast<unsigned long long>(d)) + 1ULL);
+}
+
 struct N3Radix {
   int count = 0;
   int r[20] = {};
@@ -115,6 +124,10 @@ struct N3PassParams {
   N3Radix rad;
   const cplx<R>* tw;             // exp(+2 pi i t / m), t < m
   const cplx<R>* tw2;            // exp(+2 pi i t / ntot), t < ntot (when twiddle)
+  // set by the launcher: division multipliers and shifts (see N3Radix) for the item
+  // decomposition (x tiles, n_sub, other.count) and for operm_n1
+  unsigned xt_m, sub_m, oc_m, op_m;
+  int xt_s, sub_s, oc_s, op_s;
 };
 
 template <typename R>
