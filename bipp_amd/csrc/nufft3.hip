@@ -815,13 +815,24 @@ __global__ __launch_bounds__(256) void n3_interp_kernel(N3InterpParams<R> p) {
     const int k0 = static_cast<int>(ceil(a2 - 0.5 * w));
     R* wd = d == 0 ? wx : (d == 1 ? wy : wz);
     int* kd = d == 0 ? kx : (d == 1 ? ky : kz);
+    // frequency k = (k0 + o) mod nf2 and, for two-pass axes (stored at (k mod n1) n2 +
+    // k / n1), its digits (k mod n1, k / n1): one division each for o = 0, then stepped
+    // with the wrap-around at nf2 = n1 n2
+    const int nf = p.nf2[d], n1 = p.n1[d], n2 = p.n2[d];
+    int k = k0 % nf;
+    if (k < 0) k += nf;
+    int kq = k / n1, kr = k - kq * n1;
 #pragma unroll
     for (int o = 0; o < w; ++o) {
       wd[o] = es_kernel<R>(static_cast<R>((k0 + o - a2) * 2.0 / w), R(p.beta2));
-      int k = (k0 + o) % p.nf2[d];
-      if (k < 0) k += p.nf2[d];
-      // two-pass axes store frequency k at (k mod n1) n2 + k / n1
-      kd[o] = p.n2[d] > 1 ? (k % p.n1[d]) * p.n2[d] + k / p.n1[d] : k;
+      kd[o] = n2 > 1 ? kr * n2 + kq : k;
+      ++k;
+      ++kr;
+      if (kr == n1) {
+        kr = 0;
+        ++kq;
+      }
+      if (k == nf) k = kr = kq = 0;
     }
     const double alpha1 = 0.5 * p.w1 * p.h1[d];
     corr *= p.h1[d] * p.h2[d] / (alpha1 * ft_lookup(p.ft, p.n_ft, p.ft_inv_delta, s * alpha1));
