@@ -497,23 +497,22 @@ __device__ __forceinline__ void ql_step(int i, R ei, R di, R& pp, R& c, R& s, RP
   const R g = c * ei;
   const R h = c * pp;
   const R x = fma(pp, pp, ei * ei);
-  // doubled operands times hr = 1/(2r): the factor 2 stays off the dependency chain
-  const R wnum2 = R(2) * fma(pp, di, -(ei * g));
-  const R pp2 = pp + pp, ei2 = ei + ei;
+  const R wnum = fma(pp, di, -(ei * g));
   R rr, hr;
   sqrt_half_rsqrt(x, rr, hr);
+  const R ir = hr + hr;  // 1/r (doubling is exact)
   R enew;
   if (GUARD) {
     const bool ok = x >= min_normal<R>();
     enew = ok ? s * rr : R(0);
-    s = ok ? ei2 * hr : R(0);
-    c = ok ? pp2 * hr : R(1);
-    pp = ok ? wnum2 * hr : di;
+    s = ok ? ei * ir : R(0);
+    c = ok ? pp * ir : R(1);
+    pp = ok ? wnum * ir : di;
   } else {
     enew = s * rr;
-    s = ei2 * hr;
-    c = pp2 * hr;
-    pp = wnum2 * hr;
+    s = ei * ir;
+    c = pp * ir;
+    pp = wnum * ir;
   }
   RPair<R> o;
   o.x = h + s * (c * g + s * di);
