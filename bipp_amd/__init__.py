@@ -333,12 +333,16 @@ class NufftSynthesis:
                 "nufft_sort_ms", "nufft_spread_ms", "nufft_fft_ms", "nufft_interp_ms")
         return {k: float(v) for k, v in zip(keys, ms)}
 
-    def last_eigenvalues(self, n_samples: int):
-        d = np.zeros((n_samples, self.n_beam), dtype=np.float64)
-        nr = np.zeros(n_samples, dtype=np.int32)
+    def last_eigenvalues(self):
+        """(eigenvalues (nS, nBeam) float64, kept-beam counts (nS,)) of every sample of the
+        last collect call."""
+        n = C.c_size_t()
+        check(lib().bipp_synthesis_last_eigenvalues(self._h, 0, None, None, C.byref(n)))
+        d = np.zeros((n.value, self.n_beam), dtype=np.float64)
+        nr = np.zeros(n.value, dtype=np.int32)
         check(lib().bipp_synthesis_last_eigenvalues(
-            self._h, d.ctypes.data_as(C.POINTER(C.c_double)),
-            nr.ctypes.data_as(C.POINTER(C.c_int32))))
+            self._h, n.value, d.ctypes.data_as(C.POINTER(C.c_double)),
+            nr.ctypes.data_as(C.POINTER(C.c_int32)), None))
         return d, nr
 
     def __del__(self) -> None:

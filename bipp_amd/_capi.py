@@ -85,7 +85,7 @@ _SIGS = {
     "bipp_synthesis_image_device": (C.c_int, [vp, C.POINTER(vp), C.POINTER(C.c_int)]),
     "bipp_synthesis_num_collected": (C.c_int, [vp, C.POINTER(sz)]),
     "bipp_synthesis_reset": (C.c_int, [vp]),
-    "bipp_synthesis_last_eigenvalues": (C.c_int, [vp, dp, C.POINTER(i32)]),
+    "bipp_synthesis_last_eigenvalues": (C.c_int, [vp, sz, dp, C.POINTER(i32), C.POINTER(sz)]),
     "bipp_synthesis_timings": (C.c_int, [vp, dp, C.c_int]),
     "bipp_synthesis_set_tolerance": (C.c_int, [vp, C.c_double]),
     "bipp_synthesis_nufft_grid": (C.c_int, [vp, C.POINTER(i32)]),

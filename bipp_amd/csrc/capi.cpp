@@ -48,6 +48,9 @@ BippError guard(F&& f) {
 void require(bool cond, BippError code, const char* msg) {
   if (!cond) fail(code, msg);
 }
+void require(bool cond, BippError code, const std::string& msg) {
+  if (!cond) fail(code, msg);
+}
 
 // Growable device allocation.  Reallocation frees the old block; hipFree synchronises the
 // device, so work still in flight on it has finished first.
@@ -1065,6 +1068,8 @@ struct BippSynthesisImpl {
     er.run(c, nS, n_ant, n_beam, wl, s, s_double, lds, stride_s, w, s_double, ldw, stride_w, xyz,
            ldxyz, stride_xyz, false, 1.0);
     BIPP_HIP(hipEventRecord(ev[1], st));
+    // a failed solve raises before anything of this batch reaches the images
+    er.check(c);
     const double* d_int = static_cast<const double*>(intervals.p);
     if (imager == 0) {
       const int mode = steer_mode();
@@ -1270,17 +1275,18 @@ struct BippSynthesisImpl {
       }
       BIPP_HIP(hipEventRecord(ev[3], st));
     }
-    er.check(c);
     accumulate_times();
-    // keep the eigenvalues of this call for inspection
-    last_count = nS;
-    last_d.resize((size_t)nS * n_beam);
-    last_nred.resize(nS);
+    // keep the eigenvalues of this chunk (appended to those of the collect call's earlier chunks)
+    const size_t o = (size_t)last_count;
+    last_count += nS;
+    last_d.resize((size_t)last_count * n_beam);
+    last_nred.resize(last_count);
     std::vector<R> tmp((size_t)nS * n_beam);
     BIPP_HIP(hipMemcpyAsync(tmp.data(), er.d_d(), sizeof(R) * tmp.size(), hipMemcpyDeviceToHost, st));
-    BIPP_HIP(hipMemcpyAsync(last_nred.data(), er.d_n_red(), sizeof(int) * nS, hipMemcpyDeviceToHost, st));
+    BIPP_HIP(hipMemcpyAsync(last_nred.data() + o, er.d_n_red(), sizeof(int) * nS,
+                            hipMemcpyDeviceToHost, st));
     c.sync();
-    for (size_t i = 0; i < tmp.size(); ++i) last_d[i] = static_cast<double>(tmp[i]);
+    for (size_t i = 0; i < tmp.size(); ++i) last_d[o * n_beam + i] = static_cast<double>(tmp[i]);
     n_collected += nS;
   }
 
@@ -1289,6 +1295,7 @@ struct BippSynthesisImpl {
                const double* xyz, long long ldxyz, long long stride_xyz) {
     const size_t es = s_double ? 16 : 8;
     const int chunk = batched ? 4096 : 1;
+    last_count = 0;
     for (int k0 = 0; k0 < nS; k0 += chunk) {
       const int m = std::min(chunk, nS - k0);
       const void* sk = static_cast<const char*>(s) + es * (size_t)k0 * stride_s;
@@ -1692,11 +1699,18 @@ BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset) {
   });
 }
 
-BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, double* d, int32_t* nBeamReduced) {
+BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, size_t capacity, double* d,
+                                          int32_t* nBeamReduced, size_t* count) {
   return guard([&] {
     require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
-    if (d) std::copy(h->last_d.begin(), h->last_d.end(), d);
-    if (nBeamReduced) std::copy(h->last_nred.begin(), h->last_nred.end(), nBeamReduced);
+    const size_t n = static_cast<size_t>(h->last_count);
+    if (count) *count = n;
+    if (!d && !nBeamReduced) return;
+    require(capacity >= n, BIPP_INVALID_PARAMETER_ERROR,
+            "capacity " + std::to_string(capacity) + " below the " + std::to_string(n) +
+                " samples of the last collect call");
+    if (d) std::copy(h->last_d.begin(), h->last_d.begin() + n * h->n_beam, d);
+    if (nBeamReduced) std::copy(h->last_nred.begin(), h->last_nred.begin() + n, nBeamReduced);
   });
 }
 

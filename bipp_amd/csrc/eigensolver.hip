@@ -1160,6 +1160,16 @@ __global__ __launch_bounds__(EigCfg<NMAX>::NT) void eig_kernel(EigParams<R> p) {
 // Flag live entries and compact kept beams (src/eigensolver.cpp:145-168).
 // One workgroup per sample.
 // --------------------------------------------------------------------------------------
+// |v|^2 as std::norm forms it: both squares and the sum rounded separately (a fused
+// multiply-add would change which entries sit above eps at the threshold)
+template <typename T>
+__device__ inline T norm_unfused(T re, T im) {
+#pragma clang fp contract(off)
+  const T a = re * re;
+  const T b = im * im;
+  return a + b;
+}
+
 template <typename TIn>
 __global__ __launch_bounds__(NT) void flag_kernel(FlagParams p) {
   extern __shared__ int flag_sh[];
@@ -1176,7 +1186,7 @@ __global__ __launch_bounds__(NT) void flag_kernel(FlagParams p) {
     bool any = false;
     for (int r = c + lane; r < nb; r += 64) {
       const cplx<TIn> v = S[r + (size_t)c * p.lds];
-      const bool live = p.mode == 0 ? (v.re * v.re + v.im * v.im > eps)
+      const bool live = p.mode == 0 ? (norm_unfused(v.re, v.im) > eps)
                                     : (fabs(v.re) >= eps || fabs(v.im) >= eps);
       if (live) {
         flag_sh[r] = 1;

@@ -134,8 +134,11 @@ BippError bipp_synthesis_image_device(BippSynthesis h, void** ptr, BippPrecision
 BippError bipp_synthesis_num_collected(BippSynthesis h, size_t* n);
 BippError bipp_synthesis_reset(BippSynthesis h);
 /* Eigenvalues (float64, nBeam per sample, ascending, zero padded) and kept-beam counts of
- * the samples of the last collect call, host arrays. */
-BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, double* d, int32_t* nBeamReduced);
+ * all samples of the last collect call, host arrays with room for `capacity` samples
+ * (BIPP_INVALID_PARAMETER_ERROR if the call had more).  *count (may be NULL) receives the
+ * number of samples; d and nBeamReduced may both be NULL to query it. */
+BippError bipp_synthesis_last_eigenvalues(BippSynthesis h, size_t capacity, double* d,
+                                          int32_t* nBeamReduced, size_t* count);
 /* GPU time per stage accumulated over collect calls, from HIP events on the context stream
  * (ms must hold 9 doubles): ms[0] eigensolve (flag, Gram, solve), ms[1] level weights /
  * operand preparation, ms[2] imaging kernels, ms[3] number of collect batches timed; for the

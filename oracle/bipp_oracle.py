@@ -56,10 +56,14 @@ def flag_beams(s: np.ndarray, real_dtype=np.float32, mode: str = "gram"):
     mode "gram": |s_rc|^2 > eps (eigh_gram, :153); mode "eigh": |Re|>=eps or |Im|>=eps
     (eigh, :62-63).  Returns (kept beam indices, nVis)."""
     eps = eps_of(real_dtype)
+    rdt = np.dtype(real_dtype).type
     nb = s.shape[0]
     low = np.tril(np.ones((nb, nb), dtype=bool))
     if mode == "gram":
-        live = (np.abs(s) ** 2 > eps) & low
+        # std::norm: re^2 + im^2 in the working precision, no square root (:153)
+        re = s.real.astype(rdt)
+        im = s.imag.astype(rdt)
+        live = (re * re + im * im > rdt(eps)) & low
     else:
         live = ((np.abs(s.real) >= eps) | (np.abs(s.imag) >= eps)) & low
     flag = live.any(axis=0) | live.any(axis=1)

@@ -40,6 +40,8 @@ def lib():
         L.bippc_max_threads.restype = C.c_int
         L.bippc_eigh_gram_c.argtypes = [C.c_float, C.c_int, vp, vp, vp, vp, ip, fp]
         L.bippc_eigh_gram_c.restype = C.c_int
+        L.bippc_eigh_gram_batch.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.bippc_eigh_gram_batch.restype = C.c_int
         L.bippc_steering_image.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp]
         L.bippc_nuft_sum_d.argtypes = [C.c_double, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         if L.bippc_init_lapack(openblas_path().encode()) != 0:
@@ -74,3 +76,33 @@ def steering_image(v: np.ndarray, weights: np.ndarray, pos_wl: np.ndarray, lmn: 
     assert img.dtype == np.float32 and img.flags.c_contiguous
     lib().bippc_steering_image(v.shape[0], v.shape[1], v.ctypes.data, w.ctypes.data, w.shape[0],
                                pos.ctypes.data, lm.shape[0], lm.ctypes.data, img.ctypes.data)
+
+
+def nuft_sum_d(alpha: float, values: np.ndarray, uvw: np.ndarray, lmn: np.ndarray,
+               out: np.ndarray) -> None:
+    """out (float64, P) += sum_j Re(c_j exp(i alpha (x u + y v + z w))), OpenMP over P."""
+    c = np.ascontiguousarray(values, dtype=np.complex128)
+    u, v, w = (np.ascontiguousarray(uvw[:, i], dtype=np.float64) for i in range(3))
+    x, y, z = (np.ascontiguousarray(lmn[:, i], dtype=np.float64) for i in range(3))
+    assert out.dtype == np.float64 and out.flags.c_contiguous
+    lib().bippc_nuft_sum_d(float(alpha), c.size, c.ctypes.data, u.ctypes.data, v.ctypes.data,
+                           w.ctypes.data, x.size, x.ctypes.data, y.ctypes.data, z.ctypes.data,
+                           out.ctypes.data)
+
+
+def eigh_gram_batch(wl, s: np.ndarray, xyz: np.ndarray):
+    """float32 eigh_gram of independent samples, OpenMP over samples with single-threaded
+    LAPACK each; s (nS, N, N) standard orientation, xyz (nS, N, 3)."""
+    ns, n = s.shape[0], s.shape[1]
+    s_cm = np.ascontiguousarray(np.swapaxes(s, 1, 2), dtype=np.complex64)
+    x_cm = np.ascontiguousarray(np.swapaxes(xyz, 1, 2), dtype=np.float32)
+    w = np.ascontiguousarray(np.broadcast_to(np.asarray(wl, np.float32), (ns,)))
+    d = np.zeros((ns, n), np.float32)
+    v = np.zeros((ns, n, n), np.complex64)   # per sample column-major: v[k].T is (N, N)
+    nr = np.zeros(ns, np.int32)
+    sc = np.zeros(ns, np.float32)
+    info = lib().bippc_eigh_gram_batch(ns, w.ctypes.data, n, s_cm.ctypes.data, x_cm.ctypes.data,
+                                       d.ctypes.data, v.ctypes.data, nr.ctypes.data, sc.ctypes.data)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"chegv info={info}")
+    return np.swapaxes(v, 1, 2), d, nr, sc

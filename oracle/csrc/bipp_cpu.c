@@ -8,6 +8,8 @@
  *                       precision): flag (|s|^2 > FLT_EPSILON, :148-159), Gram
  *                       sinc(2|p_i-p_j|/wl) (src/host/gram_matrix.cpp:23-56), LAPACK chegv
  *                       itype=1 'V' 'L' (src/host/lapack_api.hpp:112-134), un-beam with W=I.
+ *   bippc_eigh_gram_batch: the same over independent samples, OpenMP over samples with
+ *                       single-threaded LAPACK per thread.
  *   bippc_steering_image: I_l(x) += sum_e w_l[e] |sum_i V_ie exp(+i 2pi p_i.x/wl)|^2, the
  *                       image of src/host/virtual_vis.cpp:19-73 + src/host/kernels/nuft_sum.cpp
  *                       in closed form; OpenMP over pixels.
@@ -115,6 +117,26 @@ int bippc_eigh_gram_c(float wl, int n, const float complex* s, const float* xyz,
   free(flag);
   free(idx);
   return info;
+}
+
+/* n_samples independent eigh_gram problems of size n (same layout as bippc_eigh_gram_c,
+ * sample k at s + k n^2, xyz + 3 k n, d + k n, v + k n^2), one single-threaded LAPACK
+ * call per OpenMP thread (the "independent problems x cores" form of the CPU baseline).
+ * Returns the first nonzero LAPACK info, 0 if none. */
+int bippc_eigh_gram_batch(int n_samples, const float* wl, int n, const float complex* s,
+                          const float* xyz, float* d, float complex* v, int* nr, float* scale) {
+  int bad = 0;
+  if (p_set_threads) p_set_threads(1);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < n_samples; ++k) {
+    const int info = bippc_eigh_gram_c(wl[k], n, s + (size_t)k * n * n, xyz + (size_t)k * 3 * n,
+                                       d + (size_t)k * n, v + (size_t)k * n * n, nr + k, scale + k);
+    if (info) {
+#pragma omp critical
+      if (!bad) bad = info;
+    }
+  }
+  return bad;
 }
 
 /* v: n_ant x n_eig column-major complex64; w: n_levels x n_eig (row-major) float;

@@ -192,3 +192,120 @@ def test_golden_cfg1_reproducible(golden):
     s = z["s"][0].T  # stored column-major per sample
     v, d, nr, sc = O.eigh_gram(z["wl"][0], s, None, z["xyz"][0].T)
     assert np.allclose(d, z["d"][0], rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- C restatement (oracle/csrc)
+# The C functions are the CPU baseline's arithmetic; each is checked against the numpy
+# oracle (itself pinned above), against LAPACK in float64 and against identities.
+def _c_oracle():
+    from oracle import cpu as ocpu
+    ocpu.lib().bippc_lapack_threads(1)
+    return ocpu
+
+
+def _c_eigh_samples(golden):
+    out = []
+    w1 = synthetic.make_workload(1, n_timesteps=1, grid=8)
+    out.append(("cfg1", w1.samples[0].wl, w1.samples[0].s, w1.samples[0].xyz))
+    w2 = synthetic.make_workload(2, n_timesteps=1, grid=8)
+    out.append(("256ant", w2.samples[0].wl, w2.samples[0].s, w2.samples[0].xyz))
+    z = golden("lofar_input.npz")
+    out.append(("lofar", float(z["wl"]), z["s"][0], z["xyz"][0]))
+    return out
+
+
+def test_c_eigh_gram_matches_lapack(golden):
+    ocpu = _c_oracle()
+    for name, wl, s, xyz in _c_eigh_samples(golden):
+        s32 = s.astype(np.complex64)
+        v, d, nr, sc = ocpu.eigh_gram_c(wl, s32, xyz)
+        vp, dp, nrp, scp = O.eigh_gram(np.float32(wl), s32, None, xyz.astype(np.float32),
+                                       single=True)
+        vd, dd, nrd, scd = O.eigh_gram(wl, s32.astype(np.complex128), None, xyz)
+        assert nr == nrp == nrd, name
+        assert sc == scp and sc == pytest.approx(scd, rel=1e-7), name
+        dmax = np.abs(dd).max()
+        # same LAPACK routine (chegv) as the numpy oracle, Gram formed in float either way
+        assert np.abs(d - dp).max() <= 1e-5 * dmax, name
+        # float32 against float64 LAPACK: absolute error of order eps32 * ||A||
+        assert np.abs(d.astype(np.float64) - dd).max() <= 2e-5 * dmax, name
+        assert np.all(d[nr:] == 0) and np.all(np.diff(d[:nr]) >= 0), name
+        # pencil invariants in float64 on the kept block: S Z = G Z diag(d), Z^H G Z = I
+        idx = np.nonzero(np.abs(v).sum(1))[0]
+        zz = v[np.ix_(idx, np.arange(nr))].astype(np.complex128)
+        sr = O._lower_to_hermitian(s32.astype(np.complex128)[np.ix_(idx, idx)])
+        g = O.sinc_matrix(xyz[idx], wl)
+        assert np.abs(zz.conj().T @ g @ zz - np.eye(nr)).max() < 1e-3, name
+        res = sr @ zz - g @ zz * d[:nr].astype(np.float64)
+        assert np.abs(res).max() < 1e-4 * dmax, name
+        ref = (vd * dd) @ vd.conj().T
+        got = (v.astype(np.complex128) * d) @ v.astype(np.complex128).conj().T
+        assert np.linalg.norm(got - ref) <= 1e-4 * np.linalg.norm(ref), name
+
+
+def test_c_steering_image_matches_direct_sum():
+    ocpu = _c_oracle()
+    r = rng(21)
+    n, k, lv = 17, 6, 3
+    xyz = synthetic.disk_positions(r, n, 300.0)
+    xyz[:, 2] = r.normal(size=n) * 5
+    wl = 1.7
+    v = (r.normal(size=(n, k)) + 1j * r.normal(size=(n, k))).astype(np.complex64)
+    w = r.normal(size=(lv, k)).astype(np.float32)
+    lmn = synthetic.make_lmn_grid(20, math.radians(12))
+    img = np.zeros((lv, lmn.shape[0]), np.float32)
+    ocpu.steering_image(v, w, xyz / wl, lmn, img)
+    uvw = O.baselines_uvw(xyz) * (2 * np.pi / wl)
+    for l in range(lv):
+        vv = O.virtual_vis(1.0, w[l].astype(np.float64), v.astype(np.complex128))
+        ref = O.nuft_sum(1.0, vv, uvw, lmn)
+        assert O.rel_l2(img[l], ref) < 2e-5
+    # a conjugated steering phase (the image mirrored through the origin) must not pass
+    bad = np.zeros_like(img)
+    ocpu.steering_image(v.conj(), w, xyz / wl, lmn, bad)
+    vv0 = O.virtual_vis(1.0, w[0].astype(np.float64), v.astype(np.complex128))
+    assert O.rel_l2(bad[0], O.nuft_sum(1.0, vv0, uvw, lmn)) > 1e-2
+
+
+def test_c_point_source_peak():
+    ocpu = _c_oracle()
+    r = rng(5)
+    xyz = synthetic.disk_positions(r, 24, 200.0)
+    wl = 2.0
+    src = np.array([[0.01, -0.02, math.sqrt(1 - 0.01 ** 2 - 0.02 ** 2)]])
+    s = synthetic.visibilities(xyz, wl, src, np.array([5.0]), 0.0)
+    v, d, nr, sc = ocpu.eigh_gram_c(wl, s, xyz)
+    lmn = synthetic.make_lmn_grid(41, math.radians(6))
+    img = np.zeros((1, lmn.shape[0]), np.float32)
+    ocpu.steering_image(v, (sc * d)[None, :].astype(np.float32), xyz / wl, lmn, img)
+    q = int(np.argmax(img[0]))
+    assert np.linalg.norm(lmn[q, :2] - src[0, :2]) < 2 * (2 * math.sin(math.radians(3)) / 40)
+
+
+def test_c_nuft_sum_double():
+    ocpu = _c_oracle()
+    r = rng(22)
+    n_in, n_out = 300, 150
+    uvw = r.normal(size=(n_in, 3)) * 200
+    c = r.normal(size=n_in) + 1j * r.normal(size=n_in)
+    lmn = synthetic.make_lmn_grid(13, math.radians(10))[:n_out]
+    out = np.full(n_out, 0.5)
+    ocpu.nuft_sum_d(0.7, c, uvw, lmn, out)
+    ref = O.nuft_sum(0.7, c, uvw, lmn) + 0.5
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_c_eigh_gram_batch_equals_single_calls():
+    ocpu = _c_oracle()
+    w3 = synthetic.make_workload(3, n_timesteps=2, n_channels=2, grid=8, n_antenna=48)
+    s = np.stack([x.s for x in w3.samples])
+    xyz = np.stack([x.xyz for x in w3.samples])
+    wl = np.array([x.wl for x in w3.samples])
+    vb, db, nrb, scb = ocpu.eigh_gram_batch(wl, s, xyz)
+    for k, x in enumerate(w3.samples):
+        v, d, nr, sc = ocpu.eigh_gram_c(x.wl, x.s, x.xyz)
+        assert nr == nrb[k] and sc == scb[k]
+        assert np.array_equal(d, db[k])
+        ref = (v.astype(np.complex128) * d) @ v.astype(np.complex128).conj().T
+        got = (vb[k].astype(np.complex128) * db[k]) @ vb[k].astype(np.complex128).conj().T
+        assert np.allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
