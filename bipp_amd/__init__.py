@@ -2,9 +2,9 @@
 
 Mirrors the pybind module of the reference (python/pybipp.cpp:285-493) for the hot path:
 `Context`, `communicator`, `Partition`, `NufftSynthesisOptions`, `eigh_gram`, `eigh`,
-`image_synthesis` (in-memory dataset / image-property objects), plus the fused
-`NufftSynthesis` collect/get pipeline.  All computation runs in libbipp_amd.so on the GPU
-through the C-ABI of include/bipp_hip.h.
+`image_synthesis` on in-memory `Dataset` / `ImageProp` / `ImageData` objects (the HDF5 file
+classes are not built), plus the fused `NufftSynthesis` collect/get pipeline.  All
+computation runs in libbipp_amd.so on the GPU through the C-ABI of include/bipp_hip.h.
 """
 from __future__ import annotations
 
@@ -19,6 +19,8 @@ from . import _capi
 from ._capi import (GenericError, call, check, lib, ptr, COMPUTE, FILTERS, IMAGERS,  # noqa: F401
                     PRECISION, PU)
 from .levels import Filter, apply_filter, centroid_to_intervals, infer_intervals  # noqa: F401
+from .dataset import (Dataset, ImageData, ImageProp, InMemoryDataset,  # noqa: F401
+                      InMemoryImageData, InMemoryImageProp)
 
 __version__ = "1.0.0-mi355x"
 
@@ -120,6 +122,12 @@ class Partition:
     def grid(dimensions) -> "Partition":
         return Partition("grid", dimensions)
 
+    def _c_args(self):
+        if self.method not in _capi.PARTITION:
+            raise _capi._EXC[3](f"unknown partition method {self.method!r}")
+        dims = (C.c_size_t * 3)(*self.dimensions)
+        return _capi.PARTITION[self.method], dims
+
 
 class NufftSynthesisOptions:
     """bipp.NufftSynthesisOptions (include/bipp/image_synthesis.hpp:55-148)."""
@@ -131,9 +139,10 @@ class NufftSynthesisOptions:
         self.normalize_image = True
         self.apply_scaling = True
         self._precision = "single"
-        # additions of this build
-        self.compute: Optional[str] = None   # "mixed" | "fp32" | "fp64"
-        self.imager = "steering"             # "steering" | "direct"
+        # additions of this build: the compute form ("mixed" | "fp32" | "fp64"; default by
+        # precision) and the image_synthesis imager ("nufft" | "direct")
+        self.compute: Optional[str] = None
+        self.imager = "nufft"
 
     @property
     def precision(self) -> str:
@@ -238,7 +247,9 @@ class NufftSynthesis:
 
     def __init__(self, ctx: Context, n_antenna: int, n_beam: int, intervals, lmn,
                  filter_name: str = "lsq", compute: str = "mixed", imager: str = "steering",
-                 batched: bool = True, tolerance: Optional[float] = None) -> None:
+                 batched: bool = True, tolerance: Optional[float] = None,
+                 partition: Optional["Partition"] = None,
+                 sample_batch_size: Optional[int] = None) -> None:
         self.ctx = ctx
         iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64).reshape(-1, 2))
         lmn = np.asfortranarray(np.asarray(lmn, dtype=np.float32))
@@ -256,11 +267,23 @@ class NufftSynthesis:
         self._h = h
         if tolerance is not None:
             check(lib().bipp_synthesis_set_tolerance(h, float(tolerance)))
+        if partition is not None:
+            m, dims = partition._c_args()
+            check(lib().bipp_synthesis_set_partition(h, m, dims))
+        if sample_batch_size:
+            check(lib().bipp_synthesis_set_sample_batch_size(h, int(sample_batch_size)))
 
     def nufft_grid(self) -> Tuple[int, int, int]:
         nf = (C.c_int32 * 3)()
         check(lib().bipp_synthesis_nufft_grid(self._h, nf))
         return tuple(nf)
+
+    def nufft_partition(self):
+        """(boxes per axis, boxes holding points) of the last NUFFT collect."""
+        dims = (C.c_int32 * 3)()
+        boxes = C.c_int32()
+        check(lib().bipp_synthesis_nufft_partition(self._h, dims, C.byref(boxes)))
+        return tuple(dims), boxes.value
 
     def collect(self, wl, s, xyz, w=None) -> None:
         """Host arrays: wl (nS,) or scalar; s (nS, nBeam, nBeam) or (nBeam, nBeam), matrices
@@ -350,3 +373,6 @@ class NufftSynthesis:
         if h is not None and h.value and not sys.is_finalizing():
             lib().bipp_synthesis_destroy(h)
             self._h = None
+
+
+from .synthesis import EigenpairImager, check_selection, image_synthesis  # noqa: E402,F401

@@ -30,6 +30,7 @@ PRECISION = {"single": 0, "double": 1}
 COMPUTE = {"fp64": 0, "fp32": 1, "mixed": 2}
 FILTERS = {"lsq": 0, "std": 1, "sqrt": 2, "inv": 3, "inv_sqrt": 4}
 IMAGERS = {"steering": 0, "direct": 1, "nufft": 2}
+PARTITION = {"auto": 0, "none": 1, "grid": 2}
 
 
 class GenericError(RuntimeError):
@@ -89,6 +90,18 @@ _SIGS = {
     "bipp_synthesis_timings": (C.c_int, [vp, dp, C.c_int]),
     "bipp_synthesis_set_tolerance": (C.c_int, [vp, C.c_double]),
     "bipp_synthesis_nufft_grid": (C.c_int, [vp, C.POINTER(i32)]),
+    "bipp_synthesis_set_partition": (C.c_int, [vp, C.c_int, C.POINTER(sz)]),
+    "bipp_synthesis_nufft_partition": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
+    "bipp_synthesis_set_sample_batch_size": (C.c_int, [vp, sz]),
+    "bipp_imager_create": (C.c_int, [vp, C.c_int, C.c_int, sz, sz, sz, sz, fp, sz, C.POINTER(vp)]),
+    "bipp_imager_destroy": (C.c_int, [vp]),
+    "bipp_imager_set_tolerance": (C.c_int, [vp, C.c_double]),
+    "bipp_imager_set_partition": (C.c_int, [vp, C.c_int, C.POINTER(sz)]),
+    "bipp_imager_set_sample_batch_size": (C.c_int, [vp, sz]),
+    "bipp_imager_add": (C.c_int, [vp, C.c_float, C.c_float, fp, sz, vp, sz, fp, sz]),
+    "bipp_imager_get": (C.c_int, [vp, sz, fp]),
+    "bipp_imager_num_added": (C.c_int, [vp, C.POINTER(sz)]),
+    "bipp_imager_nufft_partition": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "bipp_virtual_vis": (C.c_int, [vp, C.c_int, sz, sz, sz, vp, sz, sz, vp, sz, vp, sz]),
     "bipp_nuft_sum": (C.c_int, [vp, C.c_int, C.c_double, sz, vp, vp, vp, vp, sz, vp, vp, vp,
                                 vp]),

@@ -67,34 +67,79 @@ __device__ __forceinline__ double ft_lookup(const double* tab, int n, double inv
 }
 
 // --------------------------------------------------------------------------------------
-// Point preparation: one thread per (sample, antenna pair a >= b).
+// Points of a slot (sample s, slot t): whether the slot carries a point, its u (radians per
+// unit direction cosine), the offset of its value in the sample's VV matrix (lower
+// triangle), whether that value is conjugated, and its multiplicity.
 // --------------------------------------------------------------------------------------
-template <typename R>
-__global__ void n3_points_kernel(N3PointsParams p) {
-  const int s = blockIdx.y;
-  const long long npair = (long long)p.n_ant_max * (p.n_ant_max + 1) / 2;
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= npair) return;
+struct SlotPoint {
+  double u[3];
+  int src;
+  int conj;
+  float mult;
+};
+
+__device__ __forceinline__ bool slot_point(const N3Source& p, int s, long long t, SlotPoint& sp) {
   const int n = p.n_rows ? p.n_rows[s] : p.n_rows_const;
-  const int a = static_cast<int>(floor((sqrt(8.0 * t + 1.0) - 1.0) * 0.5));
-  int a2 = a;
-  // guard rounding of the square root
-  if ((long long)(a2 + 1) * (a2 + 2) / 2 <= t) ++a2;
-  if ((long long)a2 * (a2 + 1) / 2 > t) --a2;
-  const int b = static_cast<int>(t - (long long)a2 * (a2 + 1) / 2);
-  const long long pid = (long long)s * npair + t;
-  if (a2 >= n || (p.merge_auto && a2 == b && a2 > 0)) {
-    p.valid[pid] = 0;
+  if (p.mode == N3_SRC_PAIRS) {
+    // pair index t = a (a + 1) / 2 + b, a >= b
+    int a = static_cast<int>(floor((sqrt(8.0 * t + 1.0) - 1.0) * 0.5));
+    if ((long long)(a + 1) * (a + 2) / 2 <= t) ++a;  // guard rounding of the square root
+    if ((long long)a * (a + 1) / 2 > t) --a;
+    const int b = static_cast<int>(t - (long long)a * (a + 1) / 2);
+    if (a >= n || (p.merge_auto && a == b && a > 0)) return false;
+    const double* pos = p.pos + (size_t)s * p.pos_stride;  // [3][apad] positions / wavelength
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      sp.u[d] = 6.283185307179586 * (pos[d * p.apad + a] - pos[d * p.apad + b]);
+    // the merged a == b point reads the trace slot (0, 1)
+    sp.src = (p.merge_auto && a == b) ? static_cast<int>(p.ld_vv) : static_cast<int>(a + p.ld_vv * b);
+    sp.conj = 0;
+    sp.mult = a == b ? 1.f : 2.f;  // (a,b) and (b,a) are conjugate: 2 Re(.)
+    return true;
+  }
+  const int N = p.n_ant_max;
+  const int j = static_cast<int>(t / N), i = static_cast<int>(t - (long long)j * N);
+  if (i >= n || j >= n) return false;
+  const bool anti = p.antisym[s] != 0;
+  if (anti && i < j) return false;                           // stands with its mirror (j, i)
+  if (anti && p.merge_auto && i == j && i > 0) return false;  // diagonal merged at (0, 0)
+  const float* uv = p.uvw + (size_t)s * p.uvw_stride + (i + (long long)N * j);
+  const double sc = p.uvw_scale[s];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sp.u[d] = sc * static_cast<double>(uv[d * p.lduvw]);
+  if (anti && p.merge_auto && i == j) {
+    sp.src = static_cast<int>(p.ld_vv);
+    sp.conj = 0;
+  } else if (i >= j) {
+    sp.src = static_cast<int>(i + p.ld_vv * j);
+    sp.conj = 0;
+  } else {
+    sp.src = static_cast<int>(j + p.ld_vv * i);  // VV(i, j) = conj(VV(j, i))
+    sp.conj = 1;
+  }
+  sp.mult = (anti && i != j) ? 2.f : 1.f;
+  return true;
+}
+
+// Point records: one thread per listed slot (or per slot of every sample).
+__global__ void n3_points_kernel(N3PointsParams p) {
+  const long long sps = p.src.slots_per_sample();
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= p.n_list) return;
+  const unsigned long long slot = p.slot_list ? p.slot_list[i] : static_cast<unsigned long long>(i);
+  const int s = static_cast<int>(slot / sps);
+  const long long t = static_cast<long long>(slot - (unsigned long long)s * sps);
+  SlotPoint sp;
+  if (!slot_point(p.src, s, t, sp)) {
+    p.valid[i] = 0;
     return;
   }
-  const double* pos = p.pos + (size_t)s * p.pos_stride;  // [3][apad] positions / wavelength
-  double u[3], cell[3];
+  double cell[3];
   N3Point pt;
   int bin[3];
   double ph = 0;
   for (int d = 0; d < 3; ++d) {
-    u[d] = 6.283185307179586 * (pos[d * p.apad + a2] - pos[d * p.apad + b]);
-    const double up = u[d] - p.C[d];
+    const double up = sp.u[d] - p.C[d];
     ph += up * p.D[d];
     cell[d] = up * p.inv_gh[d];  // t / h1 in grid units
     if (p.phase_fp32) cell[d] = static_cast<double>(static_cast<float>(up) * static_cast<float>(p.inv_gh[d]));
@@ -108,9 +153,9 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   double ps, pc;
   if (p.phase_fp32) {
     float fs, fc;
-    const float fph = static_cast<float>(u[0] - p.C[0]) * static_cast<float>(p.D[0]) +
-                      static_cast<float>(u[1] - p.C[1]) * static_cast<float>(p.D[1]) +
-                      static_cast<float>(u[2] - p.C[2]) * static_cast<float>(p.D[2]);
+    const float fph = static_cast<float>(sp.u[0] - p.C[0]) * static_cast<float>(p.D[0]) +
+                      static_cast<float>(sp.u[1] - p.C[1]) * static_cast<float>(p.D[1]) +
+                      static_cast<float>(sp.u[2] - p.C[2]) * static_cast<float>(p.D[2]);
     sincosf(fph, &fs, &fc);
     ps = fs;
     pc = fc;
@@ -122,15 +167,109 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
   pt.ph[0] = static_cast<float>(pc);
   pt.ph[1] = static_cast<float>(ps);
-  // VV(a, b) inside the sample's matrix; the merged a == b point reads the trace slot (0, 1)
-  pt.src = (p.merge_auto && a2 == b) ? static_cast<int>(p.ld_vv) : static_cast<int>(a2 + p.ld_vv * b);
-  pt.mult = a2 == b ? 1.f : 2.f;                 // (a,b) and (b,a) are conjugate: 2 Re(.)
+  pt.src = sp.src;
+  pt.mult = sp.mult;
   pt.sample = s;
-  pt.pad = 0;
-  p.valid[pid] = 1;
-  p.bin_of[pid] = binid;
-  p.pts[pid] = pt;
+  pt.flags = sp.conj;
+  p.valid[i] = 1;
+  p.bin_of[i] = binid;
+  p.pts[i] = pt;
   atomicAdd(&p.bin_count[binid], 1u);
+}
+
+// One workgroup per sample: is its uvw antisymmetric with a zero diagonal?  (Exact float
+// comparisons: baselines formed as p_i - p_j satisfy it exactly.)
+__global__ __launch_bounds__(256) void uvw_antisym_kernel(N3Source p, unsigned char* out) {
+  const int s = blockIdx.x;
+  const int N = p.n_ant_max;
+  const float* uv = p.uvw + (size_t)s * p.uvw_stride;
+  int bad = 0;
+  for (long long t = threadIdx.x; t < (long long)N * N && !bad; t += blockDim.x) {
+    const int j = static_cast<int>(t / N), i = static_cast<int>(t - (long long)j * N);
+    if (i < j) continue;
+    for (int d = 0; d < 3; ++d) {
+      const float a = uv[i + (long long)N * j + d * p.lduvw];
+      const float b = uv[j + (long long)N * i + d * p.lduvw];
+      if (i == j ? a != 0.0f : a != -b) bad = 1;
+    }
+  }
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) out[s] = bad ? 0 : 1;
+}
+
+// min / max of u over the points of item k (a sample, or a listed range of slots).
+__global__ __launch_bounds__(256) void slot_range_kernel(N3Source p,
+                                                         const unsigned long long* list,
+                                                         const unsigned* begin,
+                                                         const unsigned* count, double* out) {
+  __shared__ double red[2][3][256];
+  const int k = blockIdx.x;
+  const long long sps = p.slots_per_sample();
+  const long long n = list ? static_cast<long long>(count[k]) : sps;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned long long slot =
+        list ? list[static_cast<long long>(begin[k]) + i] : (unsigned long long)k * sps + i;
+    const int s = static_cast<int>(slot / sps);
+    SlotPoint sp;
+    if (!slot_point(p, s, static_cast<long long>(slot - (unsigned long long)s * sps), sp)) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], sp.u[d]);
+      hi[d] = fmax(hi[d], sp.u[d]);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    red[0][d][threadIdx.x] = lo[d];
+    red[1][d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        red[0][d][threadIdx.x] = fmin(red[0][d][threadIdx.x], red[0][d][threadIdx.x + off]);
+        red[1][d][threadIdx.x] = fmax(red[1][d][threadIdx.x], red[1][d][threadIdx.x + off]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    out[6 * (size_t)k + threadIdx.x] = red[0][threadIdx.x][0];
+    out[6 * (size_t)k + 3 + threadIdx.x] = red[1][threadIdx.x][0];
+  }
+}
+
+// Box of every slot's point (-1 without a point) and the point count of every box.
+__global__ void n3_partition_kernel(N3PartParams p) {
+  const long long sps = p.src.slots_per_sample();
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= sps * p.n_samples) return;
+  const int s = static_cast<int>(i / sps);
+  SlotPoint sp;
+  if (!slot_point(p.src, s, i - (long long)s * sps, sp)) {
+    p.group_of[i] = -1;
+    return;
+  }
+  int g = 0;
+  for (int d = 2; d >= 0; --d) {
+    const double x = p.inv[d] * (sp.u[d] - p.lo[d]);
+    const int k = x > 0.0 ? static_cast<int>(fmin(x, static_cast<double>(p.dims[d] - 1))) : 0;
+    g = g * p.dims[d] + k;
+  }
+  p.group_of[i] = g;
+  atomicAdd(&p.group_count[g], 1u);
+}
+
+// slot ids in box order (counting sort)
+__global__ void n3_group_scatter_kernel(long long n_slots, const int* group_of,
+                                        const unsigned* group_start, unsigned* group_fill,
+                                        unsigned long long* list) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n_slots) return;
+  const int g = group_of[i];
+  if (g < 0) return;
+  list[group_start[g] + atomicAdd(&group_fill[g], 1u)] = static_cast<unsigned long long>(i);
 }
 
 // exclusive scan helpers (counts -> starts)
@@ -290,7 +429,10 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
         const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
 #pragma unroll
         for (int lv = 0; lv < SP_LG; ++lv)
-          if (lv < L) vals[lv] = vv[(size_t)lv * p.vv_stride + pt.src];
+          if (lv < L) {
+            vals[lv] = vv[(size_t)lv * p.vv_stride + pt.src];
+            if (pt.flags & 1) vals[lv].im = -vals[lv].im;
+          }
       }
     }
     const unsigned long long mask = __ballot(hit);
@@ -1009,16 +1151,41 @@ template void launch_zero<float>(float*, long long, hipStream_t);
 template void launch_zero<double>(double*, long long, hipStream_t);
 template void launch_zero<unsigned>(unsigned*, long long, hipStream_t);
 
-template <typename R>
 void launch_n3_points(const N3PointsParams& p, int n_samples, hipStream_t st) {
-  const long long npair = (long long)p.n_ant_max * (p.n_ant_max + 1) / 2;
-  if (n_samples <= 0 || npair <= 0) return;
-  dim3 grid(static_cast<unsigned>((npair + 255) / 256), n_samples);
-  n3_points_kernel<R><<<grid, 256, 0, st>>>(p);
+  if (n_samples <= 0 || p.n_list <= 0) return;
+  if (!p.slot_list && p.n_list != p.src.slots_per_sample() * n_samples)
+    fail(BIPP_INTERNAL_ERROR, "NUFFT point slots do not match the batch");
+  n3_points_kernel<<<static_cast<unsigned>((p.n_list + 255) / 256), 256, 0, st>>>(p);
   check_launch("n3_points_kernel");
 }
-template void launch_n3_points<float>(const N3PointsParams&, int, hipStream_t);
-template void launch_n3_points<double>(const N3PointsParams&, int, hipStream_t);
+
+void launch_uvw_antisym(const N3Source& src, int n_samples, unsigned char* out, hipStream_t st) {
+  if (n_samples <= 0) return;
+  uvw_antisym_kernel<<<n_samples, 256, 0, st>>>(src, out);
+  check_launch("uvw_antisym_kernel");
+}
+
+void launch_slot_range(const N3Source& src, int n_items, const unsigned long long* list,
+                       const unsigned* begin, const unsigned* count, double* out, hipStream_t st) {
+  if (n_items <= 0) return;
+  slot_range_kernel<<<n_items, 256, 0, st>>>(src, list, begin, count, out);
+  check_launch("slot_range_kernel");
+}
+
+void launch_n3_partition(const N3PartParams& p, hipStream_t st) {
+  const long long n = p.src.slots_per_sample() * p.n_samples;
+  if (n <= 0) return;
+  n3_partition_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(p);
+  check_launch("n3_partition_kernel");
+}
+
+void launch_n3_group_scatter(long long n_slots, const int* group_of, const unsigned* group_start,
+                             unsigned* group_fill, unsigned long long* list, hipStream_t st) {
+  if (n_slots <= 0) return;
+  n3_group_scatter_kernel<<<static_cast<unsigned>((n_slots + 255) / 256), 256, 0, st>>>(
+      n_slots, group_of, group_start, group_fill, list);
+  check_launch("n3_group_scatter_kernel");
+}
 
 void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStream_t st) {
   if (n <= 0) return;

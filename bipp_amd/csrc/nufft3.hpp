@@ -26,37 +26,75 @@ struct N3Radix {
   int nb_s[20] = {}, ns_s[20] = {};
 };
 
-// One non-uniform point of the spread (one antenna pair a >= b of one sample).
+// One non-uniform point of the spread.
 struct alignas(16) N3Point {
   int l0[3];    // first cell of the kernel support, spread index + nf1/2
-  int src;      // offset of VV(a, b) inside the sample's matrix
+  int src;      // offset of the value VV(a, b) inside the sample's matrix (lower triangle)
   float fr[3];  // cell - (l0 - nf1/2) per axis, in [w/2, w/2 + 1)
-  float mult;   // 2 for a > b ((a,b) and (b,a) are conjugate), 1 for a == b
+  float mult;   // 2 for a pair whose mirror point is its conjugate, else 1
   float ph[2];  // pre-phase exp(i u'.D)
   int sample;
-  int pad;
+  int flags;    // bit 0: the value is conj(VV(src)) (an upper-triangle entry)
+};
+
+// Where the input points of a batch come from.
+//   N3_SRC_PAIRS: antenna pairs a >= b of each sample, u = 2 pi (p_a - p_b) from positions
+//     in wavelengths; slot t of a sample is the pair index a (a + 1) / 2 + b.
+//   N3_SRC_UVW:   every entry (i, j) of the sample's nAnt^2 x 3 uvw array (metres, scaled by
+//     2 pi / wl), slot t = i + nAnt j, valued VV(i, j).  Where the sample's uvw is
+//     antisymmetric (uvw(j, i) = -uvw(i, j), zero diagonal: baselines p_i - p_j), the pairs
+//     i > j stand for both entries (multiplicity 2) and the diagonal is one point at 0.
+enum { N3_SRC_PAIRS = 0, N3_SRC_UVW = 1 };
+
+struct N3Source {
+  int mode = N3_SRC_PAIRS;
+  int n_ant_max = 0;                  // rows per sample
+  const int* n_rows = nullptr;        // active rows per sample, or nullptr -> n_rows_const
+  int n_rows_const = 0;
+  const double* pos = nullptr;        // pairs: [nS][3][apad] positions / wavelength
+  long long pos_stride = 0;
+  int apad = 0;
+  const float* uvw = nullptr;         // uvw: [nS] nAnt^2 x 3 column-major (ld lduvw), metres
+  long long uvw_stride = 0, lduvw = 0;
+  const double* uvw_scale = nullptr;  // [nS] 2 pi / wl
+  const unsigned char* antisym = nullptr;  // [nS] uvw antisymmetric with zero diagonal
+  // all diagonal pairs sit at u = 0: one point per sample valued by the trace of VV_l,
+  // stored in the unused slot (0, 1) of each matrix
+  int merge_auto = 0;
+  long long ld_vv = 0;
+  __host__ __device__ long long slots_per_sample() const {
+    return mode == N3_SRC_PAIRS ? (long long)n_ant_max * (n_ant_max + 1) / 2
+                                : (long long)n_ant_max * n_ant_max;
+  }
 };
 
 struct N3PointsParams {
-  int n_ant_max;                 // rows per sample (pairs a >= b < n_ant_max)
-  const int* n_rows;             // active rows per sample, or nullptr -> n_rows_const
-  int n_rows_const;
-  const double* pos;             // [nS][3][apad] positions / wavelength
-  long long pos_stride;
-  int apad;
+  N3Source src;
+  // slots to process: slot_list[i] (= sample * slots_per_sample + t) for i < n_list, or all
+  // slots of all samples when slot_list is null
+  const unsigned long long* slot_list;
+  long long n_list;
   double C[3], D[3], inv_gh[3];  // centres and 1/(gamma h1)
   int w;
   int off[3], bin_size[3], nbins[3];
   int phase_fp32;
-  long long ld_vv;
-  // all pairs a == b sit at u = 0: emit one point per sample, valued by the trace of VV_l
-  // stored in the unused slot (0, 1) of each matrix
-  int merge_auto;
-  // outputs, [nS * npair]
+  // outputs, [n_list]
   unsigned char* valid;
   int* bin_of;
   N3Point* pts;
   unsigned* bin_count;
+};
+
+// Domain partition of the points over a grid of boxes in u (the uvw-space partition of
+// src/host/domain_partition.hpp:35-101): box index per axis = floor(inv (u - lo)) clamped to
+// [0, dims - 1], boxes numbered x fastest.
+struct N3PartParams {
+  N3Source src;
+  int n_samples;
+  int dims[3];
+  double lo[3], inv[3];
+  int* group_of;                  // [nS * sps]: box of each slot, -1 for no point
+  unsigned* group_count;          // [n_groups]
 };
 
 // Spread with output ownership: the workgroup of block b accumulates, in LDS, every point
@@ -163,9 +201,17 @@ struct N3VVParams {
 
 void launch_pos_range(int n_samples, const double* pos, long long pos_stride, int apad,
                       const int* n_rows, int n_rows_const, double* out, hipStream_t st);
+// 1 per sample whose nAnt^2 x 3 uvw is antisymmetric with a zero diagonal, else 0
+void launch_uvw_antisym(const N3Source& src, int n_samples, unsigned char* out, hipStream_t st);
+// min and max of u over the points of each item: item k covers the slots
+// list[begin[k] .. begin[k] + count[k]) (list == nullptr: slots of sample k); out [k][6]
+void launch_slot_range(const N3Source& src, int n_items, const unsigned long long* list,
+                       const unsigned* begin, const unsigned* count, double* out, hipStream_t st);
+void launch_n3_partition(const N3PartParams& p, hipStream_t st);
+void launch_n3_group_scatter(long long n_slots, const int* group_of, const unsigned* group_start,
+                             unsigned* group_fill, unsigned long long* list, hipStream_t st);
 template <typename R>
 void launch_zero(R* p, long long n, hipStream_t st);
-template <typename R>
 void launch_n3_points(const N3PointsParams& p, int n_samples, hipStream_t st);
 void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStream_t st);
 void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,

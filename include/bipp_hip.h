@@ -149,8 +149,64 @@ BippError bipp_synthesis_timings(BippSynthesis h, double* ms, int reset);
 /* NUFFT tolerance of imager 2 (NufftSynthesisOptions::tolerance,
  * include/bipp/image_synthesis.hpp:61-65); default 1e-5. */
 BippError bipp_synthesis_set_tolerance(BippSynthesis h, double tol);
-/* Inner FFT grid (nf2 per axis) of the last NUFFT collect, for inspection. */
+/* Inner FFT grid (nf2 per axis) of the last NUFFT collect (its first uvw box), for
+ * inspection. */
 BippError bipp_synthesis_nufft_grid(BippSynthesis h, int32_t* nf);
+
+/* uvw-domain partition of the type-3 NUFFT -- replaces NufftSynthesisOptions::
+ * localUVWPartition (include/bipp/image_synthesis.hpp:23-53,55-148; applied at
+ * src/gpu/nufft.hpp:110-125).  The points of a batch are split into a grid of boxes in uvw,
+ * each transformed by a plan of its own extent:
+ *   BIPP_PARTITION_AUTO: the fewest boxes (grown per axis as src/nufft_util.hpp:12-50 does)
+ *     for which one box's plan fits the FFT limits and two thirds of the device memory;
+ *   BIPP_PARTITION_NONE: one plan (an error if it exceeds the limits);
+ *   BIPP_PARTITION_GRID: dims[0] x dims[1] x dims[2] boxes. */
+enum { BIPP_PARTITION_AUTO = 0, BIPP_PARTITION_NONE = 1, BIPP_PARTITION_GRID = 2 };
+BippError bipp_synthesis_set_partition(BippSynthesis h, int method, const size_t* dims);
+/* Boxes per axis and boxes holding points of the last NUFFT collect. */
+BippError bipp_synthesis_nufft_partition(BippSynthesis h, int32_t* dims, int32_t* boxes);
+/* Samples per batch (NufftSynthesisOptions::sampleBatchSize, src/nufft_synthesis.cpp:97-103);
+ * 0: automatic (4096; the NUFFT imager min(4096, 2e9 / (20 nAntenna^2))). */
+BippError bipp_synthesis_set_sample_batch_size(BippSynthesis h, size_t n);
+
+/* ------------------------------------------------------------------------------------ *
+ * Eigenpair imager -- stage B of bipp::image_synthesis on eigenpairs a dataset holds
+ * (include/bipp/image_synthesis.hpp:160-163, src/image_synthesis.cpp:28-182): the per-sample
+ * loop of nufft_synthesis (src/nufft_synthesis.cpp:74-170) behind NUFFTInterface::add /
+ * get_image (src/nufft_interface.hpp:13-21).  The reference's add() takes virtual
+ * visibilities formed on the host; this one takes the sample's eigenpairs as Dataset hands
+ * them out (include/bipp/dataset.hpp:14-41) and forms them on the GPU.
+ *   imager: 1 = direct sum (src/host/kernels/nuft_sum.cpp:9-31), 2 = type-3 NUFFT.
+ *   compute: FP64 (double everywhere), MIXED (double virtual visibilities and point
+ *            coordinates, float grid), FP32.
+ *   lmn: nPixel x 3 host float32 (ldlmn), ImageProp::pixel_lmn (include/bipp/image_prop.hpp:20).
+ * Samples are staged in pinned memory and imaged a batch at a time (sample batch size as in
+ * bipp_synthesis_set_sample_batch_size); get() images what is staged first.
+ * ------------------------------------------------------------------------------------ */
+typedef struct BippImagerImpl* BippImager;
+
+BippError bipp_imager_create(BippContext ctx, BippCompute compute, int imager, size_t nAntenna,
+                             size_t nBeam, size_t nImages, size_t nPixel, const float* lmn,
+                             size_t ldlmn, BippImager* out);
+BippError bipp_imager_destroy(BippImager h);
+/* NufftSynthesisOptions::tolerance of the type-3 NUFFT; default 1e-5. */
+BippError bipp_imager_set_tolerance(BippImager h, double tol);
+/* As bipp_synthesis_set_partition. */
+BippError bipp_imager_set_partition(BippImager h, int method, const size_t* dims);
+/* As bipp_synthesis_set_sample_batch_size; only while no sample is staged. */
+BippError bipp_imager_set_sample_batch_size(BippImager h, size_t n);
+/* One sample, host arrays: uvw nAntenna^2 x 3 float (lduvw) in metres, unscaled (entry
+ * i + nAntenna j is the baseline of antennas i, j; the imager scales by 2 pi / wl,
+ * src/nufft_synthesis.cpp:149-155); v nAntenna x nBeam complex64 (ldv); dScaled: for each
+ * of the nImages images nBeam floats (lddScaled apart), the selected, filtered eigenvalues
+ * (zero: not used, src/host/virtual_vis.cpp:36-42); scale: the sample's 1/nVis. */
+BippError bipp_imager_add(BippImager h, float wl, float scale, const float* uvw, size_t lduvw,
+                          const void* v, size_t ldv, const float* dScaled, size_t lddScaled);
+/* Image imgIdx (nPixel floats, host) summed over every sample added, not normalised. */
+BippError bipp_imager_get(BippImager h, size_t imgIdx, float* image);
+BippError bipp_imager_num_added(BippImager h, size_t* n);
+/* Boxes per axis and boxes holding points of the last NUFFT batch. */
+BippError bipp_imager_nufft_partition(BippImager h, int32_t* dims, int32_t* boxes);
 
 /* ------------------------------------------------------------------------------------ *
  * Kernel-level entry points on device arrays (parity tests and custom pipelines).

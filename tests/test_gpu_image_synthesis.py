@@ -1,0 +1,162 @@
+"""image_synthesis on eigenpairs and uvw (stage B of the reference, src/image_synthesis.cpp:28-182,
+src/nufft_synthesis.cpp:74-170) through the eigenpair imager of the C-ABI.
+
+The LOFAR cases mirror tests/test_nufft_synthesis_lofar.cpp: eigenpairs from eigh_gram, the
+fixture's uvw rescaled to metres (:162-169), interval masks (:112-125), precision single or
+double, images compared per pixel.  The oracle is the float64 direct sum of the same stored
+(float32) dataset values (src/host/kernels/nuft_sum.cpp:9-31 over every baseline).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import bipp_amd
+from bipp_amd import synthetic
+from oracle import bipp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def lofar_dataset(golden, eig_from_gpu=True):
+    z = golden("lofar_input.npz")
+    wl = np.float32(z["wl"])
+    n_ant, n_beam = int(z["n_antenna"]), int(z["n_beam"])
+    ds = bipp_amd.InMemoryDataset.create("lofar", n_ant, n_beam)
+    for k in range(z["s"].shape[0]):
+        if eig_from_gpu:
+            v, d, sc = bipp_amd.eigh_gram(wl, z["s"][k].astype(np.complex64),
+                                          z["w"][k].astype(np.complex64),
+                                          z["xyz"][k].astype(np.float32))
+        else:
+            v, d, _, sc = O.eigh_gram(wl, z["s"][k].astype(np.complex64),
+                                      z["w"][k].astype(np.complex64),
+                                      z["xyz"][k].astype(np.float32), single=True)
+        # the reference test stores metres: uvw * wl / 2pi in float (:162-169)
+        uvw = (z["uvw"][k].astype(np.float32) * np.float32(wl / np.float32(2 * np.pi)))
+        ds.write(float(k), float(wl), float(sc), v, d, uvw)
+    return ds, z
+
+
+def masked_selection(ds, intervals, n_eig):
+    """eigMaskFunc of tests/test_nufft_synthesis_lofar.cpp:112-125."""
+    sel = {}
+    for i, (lo, hi) in enumerate(intervals):
+        tag = f"image_{i}"
+        sel[tag] = {}
+        for k in range(ds.num_samples()):
+            d = ds.eig_val(k)
+            lo32, hi32 = np.float32(lo), np.float32(hi)
+            m = np.zeros_like(d)
+            m[:n_eig] = d[:n_eig] * ((d[:n_eig] >= lo32) & (d[:n_eig] <= hi32))
+            sel[tag][k] = m
+    return sel
+
+
+def oracle_images(ds, sel, lmn):
+    """float64 direct sum over all nAntenna^2 baselines of the stored float32 values."""
+    names = list(sel)
+    out = np.zeros((len(names), lmn.shape[0]))
+    for i, name in enumerate(names):
+        for k, dsc in sel[name].items():
+            vv = O.virtual_vis(float(ds.scale(k)), np.asarray(dsc, np.float64),
+                               ds.eig_vec(k).astype(np.complex128))
+            uvw = ds.uvw(k).astype(np.float64) * (2 * np.pi / ds.wl(k))
+            out[i] += O.nuft_sum(1.0, vv, uvw, lmn)
+        out[i] /= len(sel[name])
+    return names, out
+
+
+@pytest.mark.parametrize("precision,imager,tol", [("single", "nufft", 1e-5),
+                                                  ("double", "nufft", 1e-7),
+                                                  ("single", "direct", 1e-5),
+                                                  ("double", "direct", 1e-9)])
+def test_lofar_image_synthesis(ctx, golden, precision, imager, tol):
+    e = golden("lofar_expected.npz")
+    ds, z = lofar_dataset(golden, eig_from_gpu=False)
+    sel = masked_selection(ds, z["intervals"], int(z["n_eig_int"]))
+    lmn = e["lmn"].astype(np.float32)
+    prop = bipp_amd.InMemoryImageProp.create(32, 32, 10.0, lmn)
+    opt = bipp_amd.NufftSynthesisOptions().set_precision(precision).set_tolerance(
+        1e-6 if precision == "single" else 1e-9)
+    opt.imager = imager
+    print(f"lofar image_synthesis precision={precision} imager={imager}", flush=True)
+    data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
+    names, ref = oracle_images(ds, sel, lmn.astype(np.float64))
+    assert sorted(data.tags()) == sorted(names)
+    for i, name in enumerate(names):
+        img = data.get(name).reshape(-1, order="F")
+        err = O.rel_l2(img, ref[i])
+        print(f"  {name}: rel L2 {err:.3e}, max abs {np.abs(img - ref[i]).max():.3e}")
+        assert err <= tol
+        assert np.abs(img - ref[i]).max() < 0.025
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_lofar_end_to_end_reference_tolerance(ctx, golden, precision):
+    """As the reference test: eigh_gram<float> on the GPU, dataset, masks, image_synthesis,
+    per-pixel tolerance 0.05 (tests/test_nufft_synthesis_lofar.cpp:220, ValueType float)
+    against the float32-LAPACK pipeline's images."""
+    e = golden("lofar_expected.npz")
+    ds, z = lofar_dataset(golden, eig_from_gpu=True)
+    sel = masked_selection(ds, z["intervals"], int(z["n_eig_int"]))
+    prop = bipp_amd.InMemoryImageProp.create(32, 32, 10.0, e["lmn"].astype(np.float32))
+    opt = bipp_amd.NufftSynthesisOptions().set_precision(precision)
+    data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
+    for i in range(z["intervals"].shape[0]):
+        img = data.get(f"image_{i}").reshape(-1, order="F")
+        ref = e["img_f32"][i]
+        print(f"  image_{i}: max abs {np.abs(img - ref).max():.3e}, rel L2 {O.rel_l2(img, ref):.3e}")
+        assert np.abs(img - ref).max() < 0.05
+        assert O.rel_l2(img, ref) < 1e-3
+
+
+def test_partitioned_image_synthesis(ctx, golden):
+    """Partition::Grid of the options (include/bipp/image_synthesis.hpp:23-53) changes how the
+    points are split between plans, not the image."""
+    e = golden("lofar_expected.npz")
+    ds, z = lofar_dataset(golden, eig_from_gpu=False)
+    sel = masked_selection(ds, z["intervals"], int(z["n_eig_int"]))
+    prop = bipp_amd.InMemoryImageProp.create(32, 32, 10.0, e["lmn"].astype(np.float32))
+    imgs = []
+    for part in (bipp_amd.Partition.none(), bipp_amd.Partition.grid([2, 3, 1])):
+        opt = bipp_amd.NufftSynthesisOptions().set_precision("double").set_tolerance(1e-9)
+        opt.set_local_uvw_partition(part)
+        data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
+        imgs.append(np.stack([data.get(f"image_{i}") for i in range(2)]))
+    assert O.rel_l2(imgs[1].ravel(), imgs[0].ravel()) < 1e-8
+
+
+def test_general_uvw_and_batches(ctx):
+    """uvw that is not antisymmetric (every baseline its own point, upper entries conjugate
+    of the stored lower triangle), a sample batch size smaller than the sample count, and
+    zero selected eigenvalues (skipped, src/host/virtual_vis.cpp:36-42), against the direct
+    sum of every entry; the fused diagonal is exercised by the antisymmetric sample."""
+    r = np.random.default_rng(12)
+    n_ant, n_beam, ns = 9, 7, 5
+    ds = bipp_amd.InMemoryDataset.create("rand", n_ant, n_beam)
+    for k in range(ns):
+        v = (r.normal(size=(n_ant, n_beam)) + 1j * r.normal(size=(n_ant, n_beam))) / 3
+        d = r.lognormal(size=n_beam)
+        if k == 0:
+            xyz = r.normal(size=(n_ant, 3)) * 30
+            uvw = O.baselines_uvw(xyz)     # antisymmetric, zero diagonal
+        else:
+            uvw = r.normal(size=(n_ant * n_ant, 3)) * 30
+        ds.write(float(k), 1.5 + 0.1 * k, 1.0 / 81, v, d, uvw)
+    lmn = synthetic.make_lmn_grid(16, math.radians(30)).astype(np.float32)
+    prop = bipp_amd.InMemoryImageProp.create(16, 16, 30.0, lmn)
+    sel = {"a": {}, "b": {}}
+    for k in range(ns):
+        d = ds.eig_val(k)
+        sel["a"][k] = np.where(np.arange(n_beam) % 2 == 0, d, 0)
+        sel["b"][k] = np.where(np.arange(n_beam) % 2 == 1, d, 0)
+    names, ref = oracle_images(ds, sel, lmn.astype(np.float64))
+    for precision, tol in (("double", 1e-8), ("single", 1e-5)):
+        opt = bipp_amd.NufftSynthesisOptions().set_precision(precision).set_tolerance(
+            1e-10 if precision == "double" else 1e-6).set_sample_batch_size(2)
+        data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
+        for i, name in enumerate(names):
+            err = O.rel_l2(data.get(name).reshape(-1, order="F"), ref[i])
+            print(f"general uvw {precision} {name}: rel L2 {err:.3e}")
+            assert err <= tol

@@ -433,3 +433,47 @@ def test_last_eigenvalues_per_sample_form(ctx, cfg1_golden):
         buf = np.zeros(37)
         bipp_amd.call("bipp_synthesis_last_eigenvalues", syn._h, 1,
                       buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 1), (3, 1, 2)])
+def test_nufft_partitioned_equals_one_plan(ctx, cfg1_golden, dims):
+    """uvw-domain partition (Partition::Grid, src/host/domain_partition.hpp:35-101): the points
+    split into boxes, each box transformed by a plan of its own extent; the sum over boxes is
+    the same image as one plan (the transform is linear in its points)."""
+    g = cfg1_golden
+    imgs = {}
+    for part in (bipp_amd.Partition.none(), bipp_amd.Partition.grid(dims)):
+        syn = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                      imager="nufft", tolerance=1e-9, partition=part)
+        syn.collect(g["wl"], g["s"], g["xyz"])
+        imgs[part.method] = syn.images()
+        got_dims, boxes = syn.nufft_partition()
+        print(f"partition {part.method}: dims {got_dims}, boxes with points {boxes}", flush=True)
+        if part.method == "grid":
+            assert got_dims == dims and 1 < boxes <= int(np.prod(dims))
+        else:
+            assert boxes == 1
+    for lvl in range(imgs["none"].shape[0]):
+        assert O.rel_l2(imgs["grid"][lvl], imgs["none"][lvl]) < 1e-7
+        assert O.rel_l2(imgs["grid"][lvl], g["img_st"][lvl]) < 1e-7
+
+
+def test_nufft_auto_partition_memory_bound(ctx, cfg1_golden, monkeypatch):
+    """The automatic partition grows the box count until one box's grids fit the memory
+    bound (src/nufft_util.hpp:12-50); the image is unchanged."""
+    g = cfg1_golden
+    syn = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                  imager="nufft", tolerance=1e-9)
+    syn.collect(g["wl"], g["s"], g["xyz"])
+    nf = syn.nufft_grid()
+    one = syn.images()
+    monkeypatch.setenv("BIPP_NUFFT_MAX_GRID_BYTES", str(int(np.prod(nf) * 4 * 16 / 5)))
+    syn2 = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                   imager="nufft", tolerance=1e-9)
+    syn2.collect(g["wl"], g["s"], g["xyz"])
+    dims, boxes = syn2.nufft_partition()
+    print("auto partition", dims, boxes, "grid", syn2.nufft_grid(), "single", nf, flush=True)
+    assert int(np.prod(dims)) > 1
+    img = syn2.images()
+    for lvl in range(img.shape[0]):
+        assert O.rel_l2(img[lvl], one[lvl]) < 1e-7
