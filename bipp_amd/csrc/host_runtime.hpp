@@ -31,9 +31,13 @@ BippError guard(F&& f) {
     g_last_error.clear();
     return BIPP_SUCCESS;
   } catch (const BippException& e) {
+    // a failed runtime call leaves its code as HIP's last error: clear it, so that the next
+    // launch check does not report it again
+    (void)hipGetLastError();
     g_last_error = e.what();
     return e.code;
   } catch (const std::bad_alloc&) {
+    (void)hipGetLastError();
     g_last_error = "host allocation failed";
     return BIPP_GENERIC_ERROR;
   } catch (const std::exception& e) {

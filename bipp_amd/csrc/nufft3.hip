@@ -555,7 +555,9 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     cplx<R>* g = p.grid + ((size_t)gz * p.nf2[1] + gy) * p.nf2[0] + gx;
 #pragma unroll
     for (int lv = 0; lv < SP_LG; ++lv)
-      if (lv < L) g[(size_t)(p.lv0 + lv) * p.grid_stride] = mk<R>(acc[cx][lv][0] * f, acc[cx][lv][1] * f);
+      if (lv < L)
+        g[(size_t)(p.lv0 - p.lv_begin + lv) * p.grid_stride] =
+            mk<R>(acc[cx][lv][0] * f, acc[cx][lv][1] * f);
   }
 }
 
@@ -1223,11 +1225,12 @@ void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
         p_all.nback[d] < 0 || p_all.nback[d] > 2)
       fail(BIPP_INTERNAL_ERROR, "NUFFT spread blocks do not cover the grid");
   // SP_LG levels per launch (the sums of a lane's 8 cells live in registers)
-  const int L = p_all.n_levels_total;
-  for (int lv0 = 0; lv0 < L; lv0 += SP_LG) {
+  if (p_all.lv_begin < 0 || p_all.lv_end > p_all.n_levels_total || p_all.lv_begin >= p_all.lv_end)
+    fail(BIPP_INTERNAL_ERROR, "NUFFT spread level range");
+  for (int lv0 = p_all.lv_begin; lv0 < p_all.lv_end; lv0 += SP_LG) {
     N3SpreadParams<R> p = p_all;
     p.lv0 = lv0;
-    p.n_levels = std::min(SP_LG, L - lv0);
+    p.n_levels = std::min(SP_LG, p_all.lv_end - lv0);
     n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), 64 * SpreadWaves<R>::value, 0, st>>>(p);
     check_launch("n3_spread_kernel");
   }

@@ -103,7 +103,9 @@ struct N3PartParams {
 template <typename R>
 struct N3SpreadParams {
   int n_levels, w;               // levels spread by this launch
-  int lv0, n_levels_total;       // first level of the group, levels stored per sample
+  int lv0, n_levels_total;       // first level of the launch, levels stored per sample
+  int lv_begin, lv_end;          // levels [lv_begin, lv_end) spread by the launcher; grid slot
+                                 // 0 holds level lv_begin
   double beta;
   int bin_size[3], nbins[3], nback[3];
   int nf1[3], nf2[3];

@@ -337,11 +337,10 @@ inline void set_partition(N3Options& o, int method, const size_t* dims) {
 // Boxes per axis for points in [lo, hi] and pixels of half-extent S (the auto partition of
 // src/gpu/nufft.hpp:110-125 with the sizing loop of src/nufft_util.hpp:12-50): while one
 // box's plan exceeds what a plan can transform, the axis furthest over its limit gets one
-// more box; then, while the grids of one box exceed max_bytes, the axis with the largest
-// product of box extent and pixel extent does.
+// more box; then, while one level's grid of a box exceeds max_bytes, the axis with the
+// largest product of box extent and pixel extent does.
 inline std::array<int, 3> n3_auto_partition(double tol, const double lo[3], const double hi[3],
-                                             const double S[3], int L, size_t elem,
-                                             double max_bytes) {
+                                             const double S[3], size_t elem, double max_bytes) {
   std::array<int, 3> g = {1, 1, 1};
   for (int iter = 0; iter < 100000; ++iter) {
     double X[3], ext[3];
@@ -364,7 +363,7 @@ inline std::array<int, 3> n3_auto_partition(double tol, const double lo[3], cons
       ++g[worst];
       continue;
     }
-    if (z.grid_cells() * L * double(elem) <= max_bytes) return g;
+    if (z.grid_cells() * double(elem) <= max_bytes) return g;
     int best = 0;
     for (int d = 1; d < 3; ++d)
       if (ext[d] * S[d] > ext[best] * S[best]) best = d;
@@ -428,11 +427,13 @@ struct N3Runner {
   int last_nf2[3] = {0, 0, 0};
   int last_dims[3] = {1, 1, 1};
   int last_boxes = 0;
-  // event pairs per stage and box: [box][sort, spread, FFT, interpolation][begin, end]
+  // event pairs per transform pass (a box, or one level group of a box):
+  // [pass][sort, spread, FFT, interpolation][begin, end]
   std::vector<hipEvent_t> evs;
-  int n_ev_boxes = 0;
+  int n_passes = 0;
   hipEvent_t ev_vv[2] = {nullptr, nullptr};
   double* h_buf = nullptr;  // pinned: ranges, counts and starts read back by the host
+  double max_bytes_ = 0;    // grid memory bound of one box (bytes)
   size_t h_cap = 0;
 
   N3Runner() = default;
@@ -453,8 +454,8 @@ struct N3Runner {
     }
     return h_buf;
   }
-  hipEvent_t box_event(int box, int stage, int edge) {
-    const size_t i = ((size_t)box * 4 + stage) * 2 + edge;
+  hipEvent_t pass_event(int pass, int stage, int edge) {
+    const size_t i = ((size_t)pass * 4 + stage) * 2 + edge;
     while (evs.size() <= i) {
       hipEvent_t e;
       BIPP_HIP(hipEventCreate(&e));
@@ -493,6 +494,13 @@ struct N3Runner {
   }
   static bool merge_auto(int rows_max) { return rows_max >= 2; }
 
+  // levels transformed together by the current plan: as many as the grid memory bound of
+  // one box holds, at least one
+  int lg_levels(int L) const {
+    const double lvl_bytes = double(plan.grid_cells) * sizeof(cplx<TG>);
+    return static_cast<int>(std::max(1.0, std::min<double>(L, max_bytes_ / lvl_bytes)));
+  }
+
   // Stage times of the last batch (ms): virtual visibilities, point binning and sort,
   // spread, FFT, interpolation.  Waits for the stream.
   void stage_ms(ContextImpl& c, double* ms) {
@@ -500,9 +508,9 @@ struct N3Runner {
     float t = 0;
     BIPP_HIP(hipEventElapsedTime(&t, ev_vv[0], ev_vv[1]));
     ms[0] += t;
-    for (int b = 0; b < n_ev_boxes; ++b)
+    for (int b = 0; b < n_passes; ++b)
       for (int k = 0; k < 4; ++k) {
-        BIPP_HIP(hipEventElapsedTime(&t, box_event(b, k, 0), box_event(b, k, 1)));
+        BIPP_HIP(hipEventElapsedTime(&t, pass_event(b, k, 0), pass_event(b, k, 1)));
         ms[1 + k] += t;
       }
   }
@@ -511,7 +519,7 @@ struct N3Runner {
              const double pix_lo[3], const double pix_hi[3], const double* lmn64, int n_pix,
              void* img, bool phase_fp32) {
     hipStream_t st = c.stream;
-    n_ev_boxes = 0;
+    n_passes = 0;
     last_boxes = 0;
     src.merge_auto = merge_auto(src.n_ant_max) ? 1 : 0;
     src.ld_vv = ld_vv;
@@ -538,23 +546,25 @@ struct N3Runner {
       S[d] = 0.5 * (pix_hi[d] - pix_lo[d]);
       D[d] = 0.5 * (pix_hi[d] + pix_lo[d]);
     }
+    // grid memory of one box: the option, else BIPP_NUFFT_MAX_GRID_BYTES, else two thirds of
+    // the device memory (src/gpu/nufft.hpp:112); levels that do not fit together are
+    // transformed one group after another
+    double maxb = opt.max_grid_bytes;
+    if (maxb <= 0) {
+      const char* e = std::getenv("BIPP_NUFFT_MAX_GRID_BYTES");
+      maxb = e ? std::atof(e) : 0.0;
+    }
+    if (maxb <= 0) {
+      size_t fr = 0, tot = 0;
+      BIPP_HIP(hipMemGetInfo(&fr, &tot));
+      maxb = double(tot) / 3 * 2;
+    }
+    max_bytes_ = maxb;
     std::array<int, 3> dims = {1, 1, 1};
     if (opt.part == N3_PART_GRID) {
       for (int d = 0; d < 3; ++d) dims[d] = std::max(1, opt.dims[d]);
     } else if (opt.part == N3_PART_AUTO) {
-      // grid memory of one box: the option, else BIPP_NUFFT_MAX_GRID_BYTES, else two thirds of
-      // the device memory (src/gpu/nufft.hpp:112)
-      double maxb = opt.max_grid_bytes;
-      if (maxb <= 0) {
-        const char* e = std::getenv("BIPP_NUFFT_MAX_GRID_BYTES");
-        maxb = e ? std::atof(e) : 0.0;
-      }
-      if (maxb <= 0) {
-        size_t fr = 0, tot = 0;
-        BIPP_HIP(hipMemGetInfo(&fr, &tot));
-        maxb = double(tot) / 3 * 2;
-      }
-      dims = n3_auto_partition(opt.tol, lo, hi, S, L, sizeof(cplx<TG>), maxb);
+      dims = n3_auto_partition(opt.tol, lo, hi, S, sizeof(cplx<TG>), maxb);
     }
     for (int d = 0; d < 3; ++d) last_dims[d] = dims[d];
     const long long n_groups = (long long)dims[0] * dims[1] * dims[2];
@@ -628,9 +638,9 @@ struct N3Runner {
     const N3Plan& pl = plan;
     for (int d = 0; d < 3; ++d)
       if (box == 0) last_nf2[d] = pl.nf2[d];
-    n_ev_boxes = box + 1;
-    auto mark = [&](int stage, int edge) { BIPP_HIP(hipEventRecord(box_event(box, stage, edge), st)); };
-    mark(0, 0);
+    const int first = n_passes;  // the sort is timed with the box's first pass
+    auto mark0 = [&](int edge) { BIPP_HIP(hipEventRecord(pass_event(first, 0, edge), st)); };
+    mark0(0);
     N3PointsParams pp;
     pp.src = src;
     pp.slot_list = list;
@@ -660,199 +670,214 @@ struct N3Runner {
     launch_zero<unsigned>(bfill, nb, st);
     N3Point* srt = sorted.as<N3Point>(n_list);
     launch_n3_scatter(n_list, pp.valid, pp.bin_of, pp.pts, bstart, bfill, srt, st);
-    mark(0, 1);
-    // spread + deconvolution into [-nf1/2, nf1/2)^3 of the nf2 grid
+    mark0(1);
+    // levels in groups whose grids fit the memory bound of one box (at least one level)
     const long long G = pl.grid_cells;
-    cplx<TG>* cg = grid.as<cplx<TG>>((size_t)L * G);
-    const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
-    hdec.resize(nd);
-    for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
-    for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
-    for (int i = 0; i < pl.nf1[2]; ++i)
-      hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
-    TG* ddec = deconv.as<TG>(nd);
-    BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
-    mark(1, 0);
-    N3SpreadParams<TG> sp;
-    sp.n_levels = L;
-    sp.lv0 = 0;
-    sp.n_levels_total = L;
-    sp.w = pl.w;
-    sp.beta = pl.beta;
-    for (int d = 0; d < 3; ++d) {
-      sp.bin_size[d] = pl.bin[d];
-      sp.nbins[d] = pl.nbins[d];
-      sp.nback[d] = pl.nback[d];
-      sp.nf1[d] = pl.nf1[d];
-      sp.nf2[d] = pl.nf2[d];
-    }
-    sp.bin_start = bstart;
-    sp.bin_count = pp.bin_count;
-    sp.pts = srt;
-    sp.vv = static_cast<const cplx<TG>*>(vv.p);
-    sp.vv_stride = vv_stride;
-    sp.fx = ddec;
-    sp.fy = ddec + pl.nf1[0];
-    sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
-    sp.grid = cg;
-    sp.grid_stride = G;
-    // BIPP_N3_PROFILE=1: per-block duration / work summary of the spread on stderr
-    const bool prof = std::getenv("BIPP_N3_PROFILE") != nullptr;
-    sp.prof = prof ? prof_buf.as<unsigned long long>((size_t)nb * 4) : nullptr;
-    if (prof) BIPP_HIP(hipMemsetAsync(sp.prof, 0, sizeof(unsigned long long) * nb * 4, st));
-    launch_n3_spread<TG, cplx<TG>>(sp, st);
-    if (prof) report_spread_profile(sp.prof, nb, pl, st);
-    mark(1, 1);
-    mark(2, 0);
-
-    // pruned FFT (sign +) along x, y, z; twiddle tables exp(+2 pi i t / n) computed in FP64
-    const int nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
-    // table segments: x, then per strided axis n1, n2 and n1 * n2
-    size_t tw_off[3][3] = {};
-    htw.clear();
-    // two-level tables: exp(+2 pi i t / n) = fine[t mod 64] * coarse[t / 64]
-    auto add_table = [&](long long n) {
-      const size_t o = htw.size();
-      auto root = [&](long long t) {
-        const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t % n) / n;
-        return mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang)));
+    const int lg_max = lg_levels(L);
+    cplx<TG>* cg = grid.as<cplx<TG>>((size_t)std::min(L, lg_max) * G);
+    for (int g0 = 0; g0 < L; g0 += lg_max) {
+      const int lg = std::min(lg_max, L - g0);
+      const int pass = n_passes++;
+      auto mark = [&](int stage, int edge) {
+        BIPP_HIP(hipEventRecord(pass_event(pass, stage, edge), st));
       };
-      for (long long t = 0; t < 64; ++t) htw.push_back(root(t));
-      for (long long s = 0; s < (n + 63) / 64; ++s) htw.push_back(root(64 * s));
-      return o;
-    };
-    tw_off[0][0] = add_table(nx);
-    for (int d = 1; d < 3; ++d) {
-      tw_off[d][0] = add_table(pl.n1[d]);
-      tw_off[d][1] = add_table(pl.n2[d]);
-      tw_off[d][2] = pl.n2[d] > 1 ? add_table((long long)pl.n1[d] * pl.n2[d]) : 0;
-    }
-    cplx<TG>* dtw = twid.as<cplx<TG>>(htw.size());
-    BIPP_HIP(hipMemcpyAsync(dtw, htw.data(), sizeof(cplx<TG>) * htw.size(),
-                            hipMemcpyHostToDevice, st));
-    // x: lines (y, z) of the input window; read x of the input window, write the output window
-    N3LineParams<TG> lp;
-    lp.grid = cg;
-    lp.grid_stride = G;
-    lp.n_levels = L;
-    lp.n = nx;
-    lp.ny = ny;
-    lp.nz = nz;
-    lp.yw = pl.in_w[1];
-    lp.zw = pl.in_w[2];
-    lp.in = pl.in_w[0];
-    lp.out = pl.out_w[0];
-    lp.rad = radices_of(nx, fft_radix_max());
-    lp.tw = dtw + tw_off[0][0];
-    launch_fft_lines<TG>(lp, st);
-    // y over (x of the output window, z of the input window); z over (x, y output windows)
-    for (int d = 1; d < 3; ++d) {
-      N3PassParams<TG> fp{};
-      fp.grid = cg;
-      fp.grid_stride = G;
-      fp.n_levels = L;
-      fp.nx = nx;
-      fp.xw = pl.out_w[0];
-      if (d == 1) {
-        fp.no = nz;
-        fp.other = pl.in_w[2];
-        fp.operm_n1 = 1;
-        fp.operm_n2 = 1;
-        fp.outer_stride = (long long)nx * ny;
-        fp.axis_stride = nx;
-      } else {
-        fp.no = ny;
-        fp.other = pl.out_w[1];
-        fp.operm_n1 = pl.n1[1];
-        fp.operm_n2 = pl.n2[1];
-        fp.outer_stride = nx;
-        fp.axis_stride = (long long)nx * ny;
+      // spread + deconvolution into [-nf1/2, nf1/2)^3 of the nf2 grid
+      const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
+      hdec.resize(nd);
+      for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
+      for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
+      for (int i = 0; i < pl.nf1[2]; ++i)
+        hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
+      TG* ddec = deconv.as<TG>(nd);
+      BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
+      if (pass != first) {
+        mark(0, 0);  // the points are sorted once per box
+        mark(0, 1);
       }
-      fp.nax = pl.nf2[d];
-      fp.in = pl.in_w[d];
-      fp.out = pl.out_w[d];
-      const int n1 = pl.n1[d], n2 = pl.n2[d];
-      if (n2 == 1) {
-        fp.m = n1;
-        fp.n_sub = 1;
-        fp.sub_stride = 0;
-        fp.estride = 1;
-        fp.in_check = 1;
-        fp.out_check = 1;
-        fp.fa = 0;
-        fp.fb = 1;
-        fp.twiddle = 0;
-        fp.ntot = n1;
-        fp.rad = radices_of(n1, fft_pass_radix_max());
-        fp.tw = dtw + tw_off[d][0];
-        fp.tw2 = nullptr;
-        launch_fft_pass<TG>(fp, st);
-      } else {
-        // n = n1 n2: transforms of length n1 over stride n2 (inputs pruned), twiddle, then
-        // transforms of length n2 over contiguous runs (outputs pruned); frequency k is
-        // stored at (k mod n1) n2 + k / n1
-        fp.m = n1;
-        fp.n_sub = n2;
-        fp.sub_stride = 1;
-        fp.estride = n2;
-        fp.in_check = 1;
-        fp.out_check = 0;
-        fp.twiddle = 1;
-        fp.ntot = (long long)n1 * n2;
-        fp.rad = radices_of(n1, fft_pass_radix_max());
-        fp.tw = dtw + tw_off[d][0];
-        fp.tw2 = dtw + tw_off[d][2];
-        launch_fft_pass<TG>(fp, st);
-        fp.m = n2;
-        fp.n_sub = n1;
-        fp.sub_stride = n2;
-        fp.estride = 1;
-        fp.in_check = 0;
-        fp.out_check = 1;
-        fp.fa = 1;
-        fp.fb = n1;
-        fp.twiddle = 0;
-        fp.rad = radices_of(n2, fft_pass_radix_max());
-        fp.tw = dtw + tw_off[d][1];
-        fp.tw2 = nullptr;
-        launch_fft_pass<TG>(fp, st);
+      mark(1, 0);
+      N3SpreadParams<TG> sp;
+      sp.n_levels = lg;
+      sp.lv0 = g0;
+      sp.n_levels_total = L;
+      sp.lv_begin = g0;
+      sp.lv_end = g0 + lg;
+      sp.w = pl.w;
+      sp.beta = pl.beta;
+      for (int d = 0; d < 3; ++d) {
+        sp.bin_size[d] = pl.bin[d];
+        sp.nbins[d] = pl.nbins[d];
+        sp.nback[d] = pl.nback[d];
+        sp.nf1[d] = pl.nf1[d];
+        sp.nf2[d] = pl.nf2[d];
       }
-    }
+      sp.bin_start = bstart;
+      sp.bin_count = pp.bin_count;
+      sp.pts = srt;
+      sp.vv = static_cast<const cplx<TG>*>(vv.p);
+      sp.vv_stride = vv_stride;
+      sp.fx = ddec;
+      sp.fy = ddec + pl.nf1[0];
+      sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
+      sp.grid = cg;
+      sp.grid_stride = G;
+      // BIPP_N3_PROFILE=1: per-block duration / work summary of the spread on stderr
+      const bool prof = std::getenv("BIPP_N3_PROFILE") != nullptr;
+      sp.prof = prof ? prof_buf.as<unsigned long long>((size_t)nb * 4) : nullptr;
+      if (prof) BIPP_HIP(hipMemsetAsync(sp.prof, 0, sizeof(unsigned long long) * nb * 4, st));
+      launch_n3_spread<TG, cplx<TG>>(sp, st);
+      if (prof) report_spread_profile(sp.prof, nb, pl, st);
+      mark(1, 1);
+      mark(2, 0);
 
-    mark(2, 1);
-    mark(3, 0);
-    // interpolation, correction, real part into the images
-    double* dft = gl.as<double>(pl.ft_tab.size());
-    BIPP_HIP(hipMemcpyAsync(dft, pl.ft_tab.data(), sizeof(double) * pl.ft_tab.size(),
-                            hipMemcpyHostToDevice, st));
-    N3InterpParams<TG> ip;
-    ip.n_pix = n_pix;
-    ip.n_levels = L;
-    ip.w1 = pl.w;
-    ip.w2 = pl.w;
-    ip.beta1 = pl.beta;
-    ip.beta2 = pl.beta;
-    for (int d = 0; d < 3; ++d) {
-      ip.gam[d] = pl.gam[d];
-      ip.D[d] = pl.D[d];
-      ip.C[d] = pl.C[d];
-      ip.h1[d] = pl.h1[d];
-      ip.h2[d] = pl.h2[d];
-      ip.nf2[d] = pl.nf2[d];
-      ip.n1[d] = pl.n1[d];
-      ip.n2[d] = pl.n2[d];
+      // pruned FFT (sign +) along x, y, z; twiddle tables exp(+2 pi i t / n) computed in FP64
+      const int nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
+      // table segments: x, then per strided axis n1, n2 and n1 * n2
+      size_t tw_off[3][3] = {};
+      htw.clear();
+      // two-level tables: exp(+2 pi i t / n) = fine[t mod 64] * coarse[t / 64]
+      auto add_table = [&](long long n) {
+        const size_t o = htw.size();
+        auto root = [&](long long t) {
+          const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t % n) / n;
+          return mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang)));
+        };
+        for (long long t = 0; t < 64; ++t) htw.push_back(root(t));
+        for (long long s = 0; s < (n + 63) / 64; ++s) htw.push_back(root(64 * s));
+        return o;
+      };
+      tw_off[0][0] = add_table(nx);
+      for (int d = 1; d < 3; ++d) {
+        tw_off[d][0] = add_table(pl.n1[d]);
+        tw_off[d][1] = add_table(pl.n2[d]);
+        tw_off[d][2] = pl.n2[d] > 1 ? add_table((long long)pl.n1[d] * pl.n2[d]) : 0;
+      }
+      cplx<TG>* dtw = twid.as<cplx<TG>>(htw.size());
+      BIPP_HIP(hipMemcpyAsync(dtw, htw.data(), sizeof(cplx<TG>) * htw.size(),
+                              hipMemcpyHostToDevice, st));
+      // x: lines (y, z) of the input window; read x of the input window, write the output window
+      N3LineParams<TG> lp;
+      lp.grid = cg;
+      lp.grid_stride = G;
+      lp.n_levels = lg;
+      lp.n = nx;
+      lp.ny = ny;
+      lp.nz = nz;
+      lp.yw = pl.in_w[1];
+      lp.zw = pl.in_w[2];
+      lp.in = pl.in_w[0];
+      lp.out = pl.out_w[0];
+      lp.rad = radices_of(nx, fft_radix_max());
+      lp.tw = dtw + tw_off[0][0];
+      launch_fft_lines<TG>(lp, st);
+      // y over (x of the output window, z of the input window); z over (x, y output windows)
+      for (int d = 1; d < 3; ++d) {
+        N3PassParams<TG> fp{};
+        fp.grid = cg;
+        fp.grid_stride = G;
+        fp.n_levels = lg;
+        fp.nx = nx;
+        fp.xw = pl.out_w[0];
+        if (d == 1) {
+          fp.no = nz;
+          fp.other = pl.in_w[2];
+          fp.operm_n1 = 1;
+          fp.operm_n2 = 1;
+          fp.outer_stride = (long long)nx * ny;
+          fp.axis_stride = nx;
+        } else {
+          fp.no = ny;
+          fp.other = pl.out_w[1];
+          fp.operm_n1 = pl.n1[1];
+          fp.operm_n2 = pl.n2[1];
+          fp.outer_stride = nx;
+          fp.axis_stride = (long long)nx * ny;
+        }
+        fp.nax = pl.nf2[d];
+        fp.in = pl.in_w[d];
+        fp.out = pl.out_w[d];
+        const int n1 = pl.n1[d], n2 = pl.n2[d];
+        if (n2 == 1) {
+          fp.m = n1;
+          fp.n_sub = 1;
+          fp.sub_stride = 0;
+          fp.estride = 1;
+          fp.in_check = 1;
+          fp.out_check = 1;
+          fp.fa = 0;
+          fp.fb = 1;
+          fp.twiddle = 0;
+          fp.ntot = n1;
+          fp.rad = radices_of(n1, fft_pass_radix_max());
+          fp.tw = dtw + tw_off[d][0];
+          fp.tw2 = nullptr;
+          launch_fft_pass<TG>(fp, st);
+        } else {
+          // n = n1 n2: transforms of length n1 over stride n2 (inputs pruned), twiddle, then
+          // transforms of length n2 over contiguous runs (outputs pruned); frequency k is
+          // stored at (k mod n1) n2 + k / n1
+          fp.m = n1;
+          fp.n_sub = n2;
+          fp.sub_stride = 1;
+          fp.estride = n2;
+          fp.in_check = 1;
+          fp.out_check = 0;
+          fp.twiddle = 1;
+          fp.ntot = (long long)n1 * n2;
+          fp.rad = radices_of(n1, fft_pass_radix_max());
+          fp.tw = dtw + tw_off[d][0];
+          fp.tw2 = dtw + tw_off[d][2];
+          launch_fft_pass<TG>(fp, st);
+          fp.m = n2;
+          fp.n_sub = n1;
+          fp.sub_stride = n2;
+          fp.estride = 1;
+          fp.in_check = 0;
+          fp.out_check = 1;
+          fp.fa = 1;
+          fp.fb = n1;
+          fp.twiddle = 0;
+          fp.rad = radices_of(n2, fft_pass_radix_max());
+          fp.tw = dtw + tw_off[d][1];
+          fp.tw2 = nullptr;
+          launch_fft_pass<TG>(fp, st);
+        }
+      }
+
+      mark(2, 1);
+      mark(3, 0);
+      // interpolation, correction, real part into the images
+      double* dft = gl.as<double>(pl.ft_tab.size());
+      BIPP_HIP(hipMemcpyAsync(dft, pl.ft_tab.data(), sizeof(double) * pl.ft_tab.size(),
+                              hipMemcpyHostToDevice, st));
+      N3InterpParams<TG> ip;
+      ip.n_pix = n_pix;
+      ip.n_levels = lg;
+      ip.w1 = pl.w;
+      ip.w2 = pl.w;
+      ip.beta1 = pl.beta;
+      ip.beta2 = pl.beta;
+      for (int d = 0; d < 3; ++d) {
+        ip.gam[d] = pl.gam[d];
+        ip.D[d] = pl.D[d];
+        ip.C[d] = pl.C[d];
+        ip.h1[d] = pl.h1[d];
+        ip.h2[d] = pl.h2[d];
+        ip.nf2[d] = pl.nf2[d];
+        ip.n1[d] = pl.n1[d];
+        ip.n2[d] = pl.n2[d];
+      }
+      ip.lmn = lmn64;
+      ip.ft = dft;
+      ip.n_ft = static_cast<int>(pl.ft_tab.size());
+      ip.ft_inv_delta = pl.ft_inv_delta;
+      ip.grid = cg;
+      ip.grid_stride = G;
+      ip.img = static_cast<TG*>(img) + (size_t)g0 * n_pix;
+      ip.scale = 1.0;
+      ip.img_ld = n_pix;
+      launch_n3_interp<TG, TG>(ip, st);
+      mark(3, 1);
     }
-    ip.lmn = lmn64;
-    ip.ft = dft;
-    ip.n_ft = static_cast<int>(pl.ft_tab.size());
-    ip.ft_inv_delta = pl.ft_inv_delta;
-    ip.grid = cg;
-    ip.grid_stride = G;
-    ip.img = img;
-    ip.scale = 1.0;
-    ip.img_ld = n_pix;
-    launch_n3_interp<TG, TG>(ip, st);
-    mark(3, 1);
   }
 };
 

@@ -125,10 +125,13 @@ def config_spec(cfg: int) -> dict:
 def make_workload(cfg: int, n_timesteps: Optional[int] = None, n_channels: Optional[int] = None,
                   grid: Optional[int] = None, seed: Optional[int] = None,
                   fov_deg: float = 5.0, sigma2: float = 0.1,
-                  n_antenna: Optional[int] = None) -> Workload:
+                  n_antenna: Optional[int] = None, channels: Optional[List[int]] = None,
+                  t_step: int = 1) -> Workload:
     """Build configuration `cfg` (1..5), optionally truncated in timesteps/channels/grid.
 
     "X km disk" is read as a disk of diameter X.  Seed default: 20261202 + cfg.
+    channels: indices into the configuration's channel list (default: the first
+    n_channels); t_step: timestep stride (timesteps 0, t_step, 2 t_step, ...).
     """
     spec = _CONFIGS[cfg]
     seed = 20261202 + cfg if seed is None else seed
@@ -136,11 +139,14 @@ def make_workload(cfg: int, n_timesteps: Optional[int] = None, n_channels: Optio
     n = spec["n"] if n_antenna is None else n_antenna
     p0 = disk_positions(rng, n, spec["diam"])
     src_xyz, flux = zenith_cap_sources(rng, spec["n_src"], 5.0)
-    freqs = spec["freqs"][: (n_channels or len(spec["freqs"]))]
+    if channels is not None:
+        freqs = [spec["freqs"][c] for c in channels]
+    else:
+        freqs = spec["freqs"][: (n_channels or len(spec["freqs"]))]
     n_t = spec["n_t"] if n_timesteps is None else n_timesteps
     g = spec["grid"] if grid is None else grid
     samples = []
-    for t in range(n_t):
+    for t in range(0, n_t * t_step, t_step):
         theta = 2 * math.pi * (30.0 * t) / 86400.0
         xyz = rotate_x(p0, theta)
         if spec["n_range"] is not None:

