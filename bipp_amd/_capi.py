@@ -75,6 +75,9 @@ _SIGS = {
     "bipp_eigh_d": (C.c_int, [C.c_double, sz, sz, vp, sz, vp, sz, dp, vp, sz, C.POINTER(sz), dp]),
     "bipp_eigh_gram_batched": (C.c_int, [vp, sz, sz, sz, C.c_int, C.c_int, C.c_int, vp, vp, sz,
                                          sz, vp, sz, sz, vp, sz, sz, vp, sz, vp, sz, sz, vp, vp]),
+    "bipp_eigh_gram_batched_noalloc": (C.c_int, [vp, sz, sz, sz, C.c_int, C.c_int, C.c_int, vp,
+                                                 vp, sz, sz, vp, sz, sz, vp, sz, sz, vp, sz, vp,
+                                                 sz, sz, vp, vp]),
     "bipp_synthesis_create": (C.c_int, [vp, C.c_int, C.c_int, sz, sz, sz, dp, C.c_int, sz, fp,
                                         sz, C.c_int, C.POINTER(vp)]),
     "bipp_synthesis_destroy": (C.c_int, [vp]),

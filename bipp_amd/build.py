@@ -20,8 +20,8 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libbipp_amd.so")
 ARCH = os.environ.get("BIPP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eigensolver.hip", "steering.hip", "nuft.hip", "nufft3.hip", "imager_prep.hip", "capi.cpp",
-           "imager.cpp"]
+SOURCES = ["eigensolver.hip", "steering.hip", "nuft.hip", "nufft3.hip", "imager_prep.hip",
+           "noalloc.hip", "capi.cpp", "imager.cpp"]
 
 
 def hipcc() -> str:

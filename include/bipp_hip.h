@@ -88,6 +88,24 @@ BippError bipp_eigh_gram_batched(BippContext ctx, size_t nSamples, size_t nAnten
                                  size_t ldxyz, size_t strideXyz, void* d, size_t strideD, void* v,
                                  size_t ldv, size_t strideV, int32_t* nBeamReduced, void* scale);
 
+/* Same as bipp_eigh_gram_batched with identity beamforming (w must be NULL), for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops): no
+ * counterpart in the reference.  Every scratch buffer comes from one 64 MiB device array of
+ * the library (one per GPU, held until the process ends), so a call needs
+ *   nSamples * (c nBeam^2 + (4 + r) nBeam + 1 KiB) + (2 c + r) nBeam^2 + 4 KiB <= 64 MiB
+ * (c: 16 bytes for compute DOUBLE, 8 for SINGLE; r: 8 or 4); larger requests fail with
+ * BIPP_INVALID_PARAMETER_ERROR.  Samples are solved in index order on as many workgroup
+ * slots as the array leaves room for.  Calls that overlap on one GPU race (they share the
+ * array); no hipMalloc, hipFree or allocating library routine is called. */
+BippError bipp_eigh_gram_batched_noalloc(BippContext ctx, size_t nSamples, size_t nAntenna,
+                                         size_t nBeam, BippPrecision sPrec, BippPrecision compute,
+                                         BippPrecision outPrec, const double* wl, const void* s,
+                                         size_t lds, size_t strideS, const void* w, size_t ldw,
+                                         size_t strideW, const double* xyz, size_t ldxyz,
+                                         size_t strideXyz, void* d, size_t strideD, void* v,
+                                         size_t ldv, size_t strideV, int32_t* nBeamReduced,
+                                         void* scale);
+
 /* ------------------------------------------------------------------------------------ *
  * Fused synthesis -- the per-sample chain of src/nufft_synthesis.cpp:74-170 with
  * stage A (src/eigensolver.cpp:129-225) fused in front, behind a collect/get interface
