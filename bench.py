@@ -36,11 +36,11 @@ PEAK_HBM_GBS = 8000.0
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (profiles/<kernel>_traffic.json: FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
-    calibration), or None when no such summary is committed."""
-    path = os.path.join(HERE, "profiles", f"{kernel}_traffic.json")
+def measured_traffic(kernel: str, config: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary of this
+    configuration (profiles/<kernel>_cfg<config>_traffic.json: FETCH_SIZE and WRITE_SIZE scaled
+    by the copy-kernel calibration), or None when no such summary is committed."""
+    path = os.path.join(HERE, "profiles", f"{kernel}_cfg{config}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -328,7 +328,7 @@ def main() -> None:
               "kernel_ms": eig_ms, "flops_model": "40 N^3 real flops per sample",
               "algorithmic_flops_per_launch": eig_flops,
               "algorithmic_bytes_per_launch": eig_bytes, "achieved_gbs": eig_gbs}
-    tr = measured_traffic("eig_kernel")
+    tr = measured_traffic("eig_kernel", a.config)
     if tr:
         rl_eig["traffic"] = tr["bytes_per_launch"]
         rl_eig["traffic_source"] = tr["source"]
@@ -364,6 +364,24 @@ def main() -> None:
                                        "(pruned), n3_interp_kernel",
                              "kernel_ms": kern_ms, "fine_grid": list(nf), "spread_grid": nf1,
                              "points": m_pts, "algorithmic_bytes_per_launch": nufft_bytes}
+        # measured HBM bytes of the chain: per-dispatch counters times dispatches per step
+        # (one spread launch per 4 levels; strided axes above 128 points take two passes)
+        launches = {"n3_points_kernel": 1, "n3_scatter_kernel": 1,
+                    "n3_spread_kernel": (L + 3) // 4, "fft_lines_kernel": 1,
+                    "fft_pass_kernel": sum(2 if x > 128 else 1 for x in nf[1:]),
+                    "n3_interp_kernel": 1}
+        per_kernel = {}
+        for k, cnt in launches.items():
+            t = measured_traffic(k, a.config)
+            if t:
+                per_kernel[k] = {"bytes_per_dispatch": t["bytes_per_launch"], "dispatches": cnt}
+        if len(per_kernel) == len(launches):
+            tot = sum(v["bytes_per_dispatch"] * v["dispatches"] for v in per_kernel.values())
+            stage_rl["nufft"]["traffic"] = tot
+            stage_rl["nufft"]["traffic_per_kernel"] = per_kernel
+            stage_rl["nufft"]["traffic_over_algorithmic"] = tot / nufft_bytes
+            stage_rl["nufft"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                                                   "summaries in profiles/")
         img_ms = stages["image_ms"]
     # the dominant kernel of the step is reported as "roofline"
     if eig_ms >= img_ms or not stage_rl:
