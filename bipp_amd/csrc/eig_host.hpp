@@ -120,7 +120,11 @@ struct EigRunner {
     BIPP_HIP(hipMemcpyAsync(d_order, h_order.data(), sizeof(int) * nS, hipMemcpyHostToDevice, st));
 
     const int per_cu = eig_blocks_per_cu<R>(nmax, s_double);
-    const int n_slots = std::max(1, std::min(nS, per_cu * ctx.n_cu));
+    int n_slots = std::max(1, std::min(nS, per_cu * ctx.n_cu));
+    // BIPP_EIG_MAX_SLOTS caps the persistent workgroups (their matrices then share the
+    // last-level cache among fewer slots)
+    if (const char* e = std::getenv("BIPP_EIG_MAX_SLOTS"))
+      if (std::atoi(e) > 0) n_slots = std::min(n_slots, std::atoi(e));
     EigParams<R> ep;
     ep.n_jobs = nS;
     ep.nmax = nmax;

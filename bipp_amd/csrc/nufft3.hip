@@ -81,11 +81,16 @@ struct SlotPoint {
 __device__ __forceinline__ bool slot_point(const N3Source& p, int s, long long t, SlotPoint& sp) {
   const int n = p.n_rows ? p.n_rows[s] : p.n_rows_const;
   if (p.mode == N3_SRC_PAIRS) {
-    // pair index t = a (a + 1) / 2 + b, a >= b
-    int a = static_cast<int>(floor((sqrt(8.0 * t + 1.0) - 1.0) * 0.5));
-    if ((long long)(a + 1) * (a + 2) / 2 <= t) ++a;  // guard rounding of the square root
-    if ((long long)a * (a + 1) / 2 > t) --a;
-    const int b = static_cast<int>(t - (long long)a * (a + 1) / 2);
+    // column-major lower triangle: t = T(b) + (a - b), T(b) = b (2 N - b + 1) / 2
+    const long long N = p.n_ant_max;
+    const double q = 2.0 * N + 1.0;
+    int b = static_cast<int>(floor(0.5 * (q - sqrt(q * q - 8.0 * static_cast<double>(t)))));
+    auto T = [&](long long c) { return c * (2 * N - c + 1) / 2; };
+    if (b < 0) b = 0;
+    if (b > N - 1) b = static_cast<int>(N - 1);
+    while (b > 0 && T(b) > t) --b;          // guard rounding of the square root
+    while (b < N - 1 && T(b + 1) <= t) ++b;
+    const int a = static_cast<int>(t - T(b)) + b;
     if (a >= n || (p.merge_auto && a == b && a > 0)) return false;
     const double* pos = p.pos + (size_t)s * p.pos_stride;  // [3][apad] positions / wavelength
 #pragma unroll
@@ -174,6 +179,23 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   p.valid[i] = 1;
   p.bin_of[i] = binid;
   p.pts[i] = pt;
+  // the point's values of every level, conjugated for an upper-triangle entry
+  const float sgn = sp.conj ? -1.f : 1.f;
+  if (p.vv_double) {
+    const cplx<double>* v = static_cast<const cplx<double>*>(p.vv) + (size_t)s * p.n_levels * p.vv_stride + sp.src;
+    cplx<double>* o = static_cast<cplx<double>*>(p.vals) + (size_t)i * p.n_levels;
+    for (int lv = 0; lv < p.n_levels; ++lv) {
+      const cplx<double> c = v[(size_t)lv * p.vv_stride];
+      o[lv] = mk<double>(c.re, sgn * c.im);
+    }
+  } else {
+    const cplx<float>* v = static_cast<const cplx<float>*>(p.vv) + (size_t)s * p.n_levels * p.vv_stride + sp.src;
+    cplx<float>* o = static_cast<cplx<float>*>(p.vals) + (size_t)i * p.n_levels;
+    for (int lv = 0; lv < p.n_levels; ++lv) {
+      const cplx<float> c = v[(size_t)lv * p.vv_stride];
+      o[lv] = mk<float>(c.re, sgn * c.im);
+    }
+  }
   atomicAdd(&p.bin_count[binid], 1u);
 }
 
@@ -294,15 +316,18 @@ __global__ void scan_add_kernel(unsigned* out, const unsigned* block_offsets, in
   if (i < n) out[i] += block_offsets[blockIdx.x];
 }
 
-// counting sort: point records copied to their bin's segment
+// counting sort: point records and their values copied to their bin's segment
 __global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, const int* bin_of,
-                                  const N3Point* pts, const unsigned* bin_start,
-                                  unsigned* bin_fill, N3Point* sorted) {
+                                  const N3Point* pts, const unsigned long long* vals,
+                                  int val_words, const unsigned* bin_start, unsigned* bin_fill,
+                                  N3Point* sorted, unsigned long long* sorted_vals) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n_pts || !valid[i]) return;
   const int b = bin_of[i];
   const unsigned slot = bin_start[b] + atomicAdd(&bin_fill[b], 1u);
   sorted[slot] = pts[i];
+  for (int k = 0; k < val_words; ++k)
+    sorted_vals[(size_t)slot * val_words + k] = vals[(size_t)i * val_words + k];
 }
 
 // --------------------------------------------------------------------------------------
@@ -373,7 +398,15 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
   const int L = p.n_levels;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   SpreadBatch<R>& sb = sh.batch[wave];
-  const int bin = blockIdx.x;
+  // Workgroups go to the 8 XCDs in turn (workgroup i to XCD i mod 8).  With xcd_chunk > 0,
+  // runs of xcd_chunk consecutive blocks go to one XCD (neighbouring blocks read overlapping
+  // point ranges and then share an L2), the runs dealt to the XCDs in turn.
+  int bin = static_cast<int>(blockIdx.x);
+  if (p.xcd_chunk > 0) {
+    const int x = bin % 8, k = bin / 8;
+    bin = ((k / p.xcd_chunk) * 8 + x) * p.xcd_chunk + k % p.xcd_chunk;
+  }
+  if (bin >= p.nbins[0] * p.nbins[1] * p.nbins[2]) return;
   const int bx = bin % p.nbins[0];
   const int by = (bin / p.nbins[0]) % p.nbins[1];
   const int bz = bin / (p.nbins[0] * p.nbins[1]);
@@ -426,13 +459,10 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
       }
       if (hit) {
         // values first: their latency overlaps the compaction and the kernel weights
-        const TV* vv = p.vv + ((size_t)pt.sample * p.n_levels_total + p.lv0) * p.vv_stride;
+        const TV* pv = p.vals + (size_t)(rbeg[r] + v) * p.n_levels_total + p.lv0;
 #pragma unroll
         for (int lv = 0; lv < SP_LG; ++lv)
-          if (lv < L) {
-            vals[lv] = vv[(size_t)lv * p.vv_stride + pt.src];
-            if (pt.flags & 1) vals[lv].im = -vals[lv].im;
-          }
+          if (lv < L) vals[lv] = pv[lv];
       }
     }
     const unsigned long long mask = __ballot(hit);
@@ -1206,11 +1236,13 @@ void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStr
 }
 
 void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const N3Point* pts, const unsigned* bin_start, unsigned* bin_fill,
-                       N3Point* sorted, hipStream_t st) {
+                       const N3Point* pts, const void* vals, int val_words,
+                       const unsigned* bin_start, unsigned* bin_fill, N3Point* sorted,
+                       void* sorted_vals, hipStream_t st) {
   if (n_pts <= 0) return;
   n3_scatter_kernel<<<static_cast<unsigned>((n_pts + 255) / 256), 256, 0, st>>>(
-      n_pts, valid, bin_of, pts, bin_start, bin_fill, sorted);
+      n_pts, valid, bin_of, pts, static_cast<const unsigned long long*>(vals), val_words,
+      bin_start, bin_fill, sorted, static_cast<unsigned long long*>(sorted_vals));
   check_launch("n3_scatter_kernel");
 }
 
@@ -1231,7 +1263,11 @@ void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {
     N3SpreadParams<R> p = p_all;
     p.lv0 = lv0;
     p.n_levels = std::min(SP_LG, p_all.lv_end - lv0);
-    n3_spread_kernel<R, TV><<<static_cast<unsigned>(n_blocks), 64 * SpreadWaves<R>::value, 0, st>>>(p);
+    // with the XCD order (see the kernel), a multiple of 8 chunks of workgroups; the extra
+    // ones exit
+    const long long unit = p.xcd_chunk > 0 ? 8LL * p.xcd_chunk : 1;
+    const long long grid = (n_blocks + unit - 1) / unit * unit;
+    n3_spread_kernel<R, TV><<<static_cast<unsigned>(grid), 64 * SpreadWaves<R>::value, 0, st>>>(p);
     check_launch("n3_spread_kernel");
   }
 }

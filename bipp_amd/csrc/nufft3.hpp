@@ -39,7 +39,9 @@ struct alignas(16) N3Point {
 
 // Where the input points of a batch come from.
 //   N3_SRC_PAIRS: antenna pairs a >= b of each sample, u = 2 pi (p_a - p_b) from positions
-//     in wavelengths; slot t of a sample is the pair index a (a + 1) / 2 + b.
+//     in wavelengths; slot t of a sample runs down the columns of the lower triangle,
+//     t = b (2 n - b + 1) / 2 + (a - b) for n = n_ant_max rows (consecutive slots read
+//     consecutive entries of the sample's VV matrices).
 //   N3_SRC_UVW:   every entry (i, j) of the sample's nAnt^2 x 3 uvw array (metres, scaled by
 //     2 pi / wl), slot t = i + nAnt j, valued VV(i, j).  Where the sample's uvw is
 //     antisymmetric (uvw(j, i) = -uvw(i, j), zero diagonal: baselines p_i - p_j), the pairs
@@ -78,10 +80,16 @@ struct N3PointsParams {
   int w;
   int off[3], bin_size[3], nbins[3];
   int phase_fp32;
+  // values: VV_l(src) of every level (conjugated for upper-triangle entries) copied next to
+  // the record, so that the spread streams them with the sorted points
+  const void* vv;                // [nS][n_levels][vv_stride] complex (vv_double: double)
+  long long vv_stride;
+  int n_levels, vv_double;
   // outputs, [n_list]
   unsigned char* valid;
   int* bin_of;
   N3Point* pts;
+  void* vals;                    // [n_list][n_levels] complex
   unsigned* bin_count;
 };
 
@@ -111,13 +119,13 @@ struct N3SpreadParams {
   int nf1[3], nf2[3];
   const unsigned *bin_start, *bin_count;
   const N3Point* pts;            // sorted by bin
-  const cplx<R>* vv;             // [nS][L][vv_stride]
-  long long vv_stride;
+  const cplx<R>* vals;           // [sorted point][n_levels_total] values
   const R *fx, *fy, *fz;         // 1/phi2^(l) for l = -nf1/2 .. nf1/2-1
   cplx<R>* grid;                 // [L][grid_stride]
   long long grid_stride;
   // optional per-block profile [n_blocks][4]: start, end (wall clock), points, items
   unsigned long long* prof;
+  int xcd_chunk;                 // blocks per XCD run (0: blocks in launch order)
 };
 
 // Window of an axis of length n: indices (start + i) mod n for i < count.
@@ -216,9 +224,11 @@ template <typename R>
 void launch_zero(R* p, long long n, hipStream_t st);
 void launch_n3_points(const N3PointsParams& p, int n_samples, hipStream_t st);
 void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStream_t st);
+// counting sort of the records and their values (val_words 8-byte words per point)
 void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const N3Point* pts, const unsigned* bin_start, unsigned* bin_fill,
-                       N3Point* sorted, hipStream_t st);
+                       const N3Point* pts, const void* vals, int val_words,
+                       const unsigned* bin_start, unsigned* bin_fill, N3Point* sorted,
+                       void* sorted_vals, hipStream_t st);
 template <typename R, typename TV>
 void launch_n3_spread(const N3SpreadParams<R>& p, hipStream_t st);
 template <typename R>

@@ -416,7 +416,7 @@ inline void report_spread_profile(const unsigned long long* dprof, long long nb,
 // level accumulated into `img` (TG per pixel).
 template <typename TG>
 struct N3Runner {
-  DevBuf grid, valid, bin_of, pts, sorted;
+  DevBuf grid, valid, bin_of, pts, sorted, pvals, sorted_vals;
   DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid, prof_buf;
   DevBuf group_of, group_count, group_start, group_fill, group_tmp, slot_list, ranges;
   std::vector<cplx<TG>> htw;
@@ -658,6 +658,11 @@ struct N3Runner {
     pp.valid = valid.as<unsigned char>(n_list);
     pp.bin_of = bin_of.as<int>(n_list);
     pp.pts = pts.as<N3Point>(n_list);
+    pp.vv = vv.p;
+    pp.vv_stride = vv_stride;
+    pp.n_levels = L;
+    pp.vv_double = sizeof(TG) == 8 ? 1 : 0;
+    pp.vals = pvals.as<cplx<TG>>((size_t)n_list * L);
     const long long nb = pl.n_bins();
     if (nb > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT bins");
     pp.bin_count = bin_count.as<unsigned>(nb);
@@ -669,7 +674,9 @@ struct N3Runner {
     unsigned* bfill = bin_fill.as<unsigned>(nb);
     launch_zero<unsigned>(bfill, nb, st);
     N3Point* srt = sorted.as<N3Point>(n_list);
-    launch_n3_scatter(n_list, pp.valid, pp.bin_of, pp.pts, bstart, bfill, srt, st);
+    cplx<TG>* svals = sorted_vals.as<cplx<TG>>((size_t)n_list * L);
+    launch_n3_scatter(n_list, pp.valid, pp.bin_of, pp.pts, pp.vals,
+                      static_cast<int>(L * sizeof(cplx<TG>) / 8), bstart, bfill, srt, svals, st);
     mark0(1);
     // levels in groups whose grids fit the memory bound of one box (at least one level)
     const long long G = pl.grid_cells;
@@ -713,8 +720,16 @@ struct N3Runner {
       sp.bin_start = bstart;
       sp.bin_count = pp.bin_count;
       sp.pts = srt;
-      sp.vv = static_cast<const cplx<TG>*>(vv.p);
-      sp.vv_stride = vv_stride;
+      sp.vals = svals;
+      // blocks per XCD run of the spread: 32 (spread 9.4 ms at configuration 4 against 11.1 ms
+      // in launch order, 9.45 ms with 16 and 9.42 ms with 64); BIPP_SPREAD_XCD_CHUNK overrides
+      {
+        static const int chunk = [] {
+          const char* e = std::getenv("BIPP_SPREAD_XCD_CHUNK");
+          return e ? std::max(0, std::atoi(e)) : 32;
+        }();
+        sp.xcd_chunk = chunk;
+      }
       sp.fx = ddec;
       sp.fy = ddec + pl.nf1[0];
       sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
