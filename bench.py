@@ -79,11 +79,12 @@ def masks(d, intervals):
 
 
 def run_form(ctx, n, dev, n_s, lmn, intervals, compute, imager, batched, steps, warmup,
-             tol=None, first=0):
-    """Time `steps` passes over samples [first, first + n_s) of the device arrays."""
+             tol=None, first=0, autotune=False):
+    """Time `steps` passes over samples [first, first + n_s) of the device arrays (with
+    autotune, the NUFFT shapes are tuned in the first warmup pass)."""
     s_d, x_d, wl_d = dev
     syn = bipp_amd.NufftSynthesis(ctx, n, n, intervals, lmn, compute=compute, imager=imager,
-                                  batched=batched, tolerance=tol)
+                                  batched=batched, tolerance=tol, autotune=autotune)
     s_ptr = s_d.ptr + first * n * n * 8
     x_ptr = x_d.ptr + first * 3 * n * 8
     w_ptr = wl_d.ptr + first * 8
@@ -214,6 +215,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) rate")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default NUFFT shapes (otherwise tuned during the warmup)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="end with one device-to-device copy of BYTES (hardware-counter "
                          "calibration for rocprofv3 --pmc runs)")
@@ -235,8 +238,10 @@ def main() -> None:
     k_eig = masks(d_all, intervals).any(axis=0).sum(axis=1)
 
     # mixed / batched form (the measured value)
+    tune = imager == "nufft" and not a.no_autotune and a.warmup > 0
     syn_m, t_m, tm_m = run_form(ctx, n, dev, n_s, wl_.lmn, intervals, "mixed", imager, True,
-                                a.steps, a.warmup, a.tol)
+                                a.steps, a.warmup, a.tol, autotune=tune)
+    tuning = syn_m.nufft_tuning() if imager == "nufft" else None
     img_m = syn_m.images()
     d_m, _ = syn_m.last_eigenvalues()
     nb = max(tm_m["batches"], 1)
@@ -425,6 +430,10 @@ def main() -> None:
         "plain_fp32": plain,
         "speedup_vs_plain": (value / plain["samples_per_s"]) if (plain and value) else None,
         "stages_ms_per_step": stages,
+        "nufft_tuning": ({"shape": tuning[0], "candidates_timed": len(tuning[1]),
+                          "fastest_ms": min((r[1] for r in tuning[1]), default=None),
+                          "slowest_ms": max((r[1] for r in tuning[1]), default=None)}
+                         if tuning else None),
         "roofline": roofline,
         "roofline_stages": stage_rl,
         "cpu_baseline": cpu,

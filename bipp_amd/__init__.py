@@ -249,7 +249,7 @@ class NufftSynthesis:
                  filter_name: str = "lsq", compute: str = "mixed", imager: str = "steering",
                  batched: bool = True, tolerance: Optional[float] = None,
                  partition: Optional["Partition"] = None,
-                 sample_batch_size: Optional[int] = None) -> None:
+                 sample_batch_size: Optional[int] = None, autotune: bool = False) -> None:
         self.ctx = ctx
         iv = np.ascontiguousarray(np.asarray(intervals, dtype=np.float64).reshape(-1, 2))
         lmn = np.asfortranarray(np.asarray(lmn, dtype=np.float32))
@@ -272,6 +272,23 @@ class NufftSynthesis:
             check(lib().bipp_synthesis_set_partition(h, m, dims))
         if sample_batch_size:
             check(lib().bipp_synthesis_set_sample_batch_size(h, int(sample_batch_size)))
+        if autotune:
+            check(lib().bipp_synthesis_set_autotune(h, 1))
+
+    def nufft_tuning(self):
+        """(shape of the last NUFFT batch as a dict, list of (candidate dict, ms) timed by
+        this handle's last tuning)."""
+        keys = ("lines_radix", "pass_radix", "pass_tw_shift", "spread_chunk")
+        shape = (C.c_int32 * 4)()
+        n = C.c_size_t()
+        check(lib().bipp_synthesis_nufft_tuning(self._h, shape, 0, None, C.byref(n)))
+        cand = np.zeros(5 * max(n.value, 1))
+        check(lib().bipp_synthesis_nufft_tuning(self._h, shape, n.value,
+                                                cand.ctypes.data_as(C.POINTER(C.c_double)),
+                                                C.byref(n)))
+        rows = [(dict(zip(keys, (int(x) for x in cand[5 * k:5 * k + 4]))), float(cand[5 * k + 4]))
+                for k in range(n.value)]
+        return dict(zip(keys, list(shape))), rows
 
     def nufft_grid(self) -> Tuple[int, int, int]:
         nf = (C.c_int32 * 3)()

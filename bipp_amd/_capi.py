@@ -96,6 +96,9 @@ _SIGS = {
     "bipp_synthesis_set_partition": (C.c_int, [vp, C.c_int, C.POINTER(sz)]),
     "bipp_synthesis_nufft_partition": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "bipp_synthesis_set_sample_batch_size": (C.c_int, [vp, sz]),
+    "bipp_synthesis_set_autotune": (C.c_int, [vp, C.c_int]),
+    "bipp_synthesis_nufft_tuning": (C.c_int, [vp, C.POINTER(i32), sz, dp, C.POINTER(sz)]),
+    "bipp_nufft_tuning_reset": (C.c_int, []),
     "bipp_imager_create": (C.c_int, [vp, C.c_int, C.c_int, sz, sz, sz, sz, fp, sz, C.POINTER(vp)]),
     "bipp_imager_destroy": (C.c_int, [vp]),
     "bipp_imager_set_tolerance": (C.c_int, [vp, C.c_double]),

@@ -728,6 +728,28 @@ BippError bipp_synthesis_nufft_grid(BippSynthesis h, int32_t* nf) {
   });
 }
 
+BippError bipp_synthesis_set_autotune(BippSynthesis h, int enable) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    h->n3f.autotune = h->n3d.autotune = enable != 0;
+  });
+}
+
+BippError bipp_synthesis_nufft_tuning(BippSynthesis h, int32_t* shape, size_t capacity,
+                                      double* candidates, size_t* n_candidates) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    const bool d64 = h->compute == BIPP_COMPUTE_FP64;
+    const N3Tuning& t = d64 ? h->n3d.last_tuning : h->n3f.last_tuning;
+    const auto& log = d64 ? h->n3d.tune_log : h->n3f.tune_log;
+    export_tuning(t, log, shape, capacity, candidates, n_candidates);
+  });
+}
+
+BippError bipp_nufft_tuning_reset(void) {
+  return guard([&] { n3_tune_clear(); });
+}
+
 BippError bipp_synthesis_nufft_partition(BippSynthesis h, int32_t* dims, int32_t* boxes) {
   return guard([&] {
     require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");

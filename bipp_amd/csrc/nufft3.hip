@@ -1327,8 +1327,9 @@ void launch_fft_pass(const N3PassParams<R>& p_in, hipStream_t st) {
     fail(BIPP_INTERNAL_ERROR, "FFT pass windows out of range");
   // x positions per item: a power of two in [16, 64] with about 4096 elements per item
   const int ld = p.m | 1;
+  const int tw_cap = p.tw_shift_max >= 4 && p.tw_shift_max <= 6 ? p.tw_shift_max : 6;
   int tw_shift = 4;
-  while (tw_shift < 6 && (ld << (tw_shift + 1)) <= 4096) ++tw_shift;
+  while (tw_shift < tw_cap && (ld << (tw_shift + 1)) <= 4096) ++tw_shift;
   const int TW = 1 << tw_shift;
   if ((long long)p.m * TW > FFT_NT * FFT_PF) fail(BIPP_INTERNAL_ERROR, "FFT pass item too large");
   const long long items =

@@ -172,6 +172,7 @@ struct N3PassParams {
   N3Radix rad;
   const cplx<R>* tw;             // exp(+2 pi i t / m), t < m
   const cplx<R>* tw2;            // exp(+2 pi i t / ntot), t < ntot (when twiddle)
+  int tw_shift_max;               // items hold at most 2^tw_shift_max x positions (4..6)
   // set by the launcher: division multipliers and shifts (see N3Radix) for the item
   // decomposition (x tiles, n_sub, other.count) and for operm_n1
   unsigned xt_m, sub_m, oc_m, op_m;

@@ -3,6 +3,8 @@
 #pragma once
 
 #include <array>
+#include <map>
+#include <mutex>
 
 #include "host_runtime.hpp"
 #include "nufft3.hpp"
@@ -47,6 +49,13 @@ int fft_radix_max() {
 }
 int fft_pass_radix_max() {
   static const int v = radix_cap("BIPP_FFT_PASS_RADIX_MAX", 4);
+  return v;
+}
+int spread_chunk_default() {
+  static const int v = [] {
+    const char* e = std::getenv("BIPP_SPREAD_XCD_CHUNK");
+    return e ? std::max(0, std::atoi(e)) : 32;
+  }();
   return v;
 }
 
@@ -307,6 +316,90 @@ struct N3Plan {
   long long n_bins() const { return (long long)nbins[0] * nbins[1] * nbins[2]; }
 };
 
+// Run-time shapes of the spread and the FFT, picked per grid shape by the autotuner:
+// largest radix of the x lines and of the strided passes (radices_of), the largest item width
+// of the strided passes (2^pass_tw_shift x positions, at most about 4096 elements per item)
+// and the spread's blocks per XCD run (0: launch order).  The defaults are the measured best
+// at configuration 4 (x lines radix 9: FFT 18.3 ms against 18.5-19.7 ms with 8 or 16; passes
+// radix 4; spread runs of 32: 9.4 ms against 11.1 ms in launch order); BIPP_FFT_RADIX_MAX,
+// BIPP_FFT_PASS_RADIX_MAX and BIPP_SPREAD_XCD_CHUNK override them.
+struct N3Tuning {
+  int lines_radix = fft_radix_max();
+  int pass_radix = fft_pass_radix_max();
+  int pass_tw_shift = 6;
+  int spread_chunk = spread_chunk_default();
+};
+
+inline std::vector<N3Tuning> n3_tune_candidates() {
+  std::vector<N3Tuning> out;
+  for (int lr : {9, 16, 8})
+    for (int pr : {4, 8})
+      for (int tw : {6, 5})
+        for (int ch : {32, 0}) {
+          N3Tuning t;
+          t.lines_radix = lr;
+          t.pass_radix = pr;
+          t.pass_tw_shift = tw;
+          t.spread_chunk = ch;
+          out.push_back(t);
+        }
+  return out;
+}
+
+inline std::string n3_tune_key(const N3Plan& pl, int L, size_t tg) {
+  return std::to_string(pl.nf2[0]) + "x" + std::to_string(pl.nf2[1]) + "x" +
+         std::to_string(pl.nf2[2]) + "/w" + std::to_string(pl.w) + "/L" + std::to_string(L) +
+         "/" + std::to_string(tg);
+}
+
+// process-wide table of tuned shapes
+inline std::mutex& n3_tune_mu() {
+  static std::mutex m;
+  return m;
+}
+inline std::map<std::string, N3Tuning>& n3_tune_table() {
+  static std::map<std::string, N3Tuning> t;
+  return t;
+}
+inline bool n3_tune_lookup(const std::string& key, N3Tuning& out) {
+  std::lock_guard<std::mutex> lock(n3_tune_mu());
+  auto it = n3_tune_table().find(key);
+  if (it == n3_tune_table().end()) return false;
+  out = it->second;
+  return true;
+}
+inline void n3_tune_store(const std::string& key, const N3Tuning& t) {
+  std::lock_guard<std::mutex> lock(n3_tune_mu());
+  n3_tune_table()[key] = t;
+}
+inline void n3_tune_clear() {
+  std::lock_guard<std::mutex> lock(n3_tune_mu());
+  n3_tune_table().clear();
+}
+
+// The C-ABI view of a tuning: shape[4] = (lines radix, pass radix, pass item shift, spread
+// chunk); candidates: 5 doubles per timed candidate (its shape and ms), capacity candidates.
+inline void export_tuning(const N3Tuning& t, const std::vector<std::pair<N3Tuning, float>>& log,
+                          int32_t* shape, size_t capacity, double* cand, size_t* n_cand) {
+  if (shape) {
+    shape[0] = t.lines_radix;
+    shape[1] = t.pass_radix;
+    shape[2] = t.pass_tw_shift;
+    shape[3] = t.spread_chunk;
+  }
+  if (n_cand) *n_cand = log.size();
+  if (cand)
+    for (size_t k = 0; k < log.size() && k < capacity; ++k) {
+      const N3Tuning& c = log[k].first;
+      double* r = cand + 5 * k;
+      r[0] = c.lines_radix;
+      r[1] = c.pass_radix;
+      r[2] = c.pass_tw_shift;
+      r[3] = c.spread_chunk;
+      r[4] = log[k].second;
+    }
+}
+
 // How the points of a batch are split into uvw boxes, each transformed by its own plan
 // (Partition of include/bipp/image_synthesis.hpp:23-53).
 enum { N3_PART_AUTO = 0, N3_PART_NONE = 1, N3_PART_GRID = 2 };
@@ -432,6 +525,10 @@ struct N3Runner {
   std::vector<hipEvent_t> evs;
   int n_passes = 0;
   hipEvent_t ev_vv[2] = {nullptr, nullptr};
+  size_t tw_off[3][3] = {};
+  bool autotune = false;
+  N3Tuning last_tuning;
+  std::vector<std::pair<N3Tuning, float>> tune_log;  // candidates and ms of the last tuning
   double* h_buf = nullptr;  // pinned: ranges, counts and starts read back by the host
   double max_bytes_ = 0;    // grid memory bound of one box (bytes)
   size_t h_cap = 0;
@@ -682,217 +779,274 @@ struct N3Runner {
     const long long G = pl.grid_cells;
     const int lg_max = lg_levels(L);
     cplx<TG>* cg = grid.as<cplx<TG>>((size_t)std::min(L, lg_max) * G);
+    upload_tables(pl, st);
+    // the run-time shapes of the spread and the FFT: the tuned ones of this grid shape, or
+    // the defaults; with autotuning on, every candidate is timed on the first level group of
+    // the first batch of a shape (the spread rewrites its whole window and the FFT reads only
+    // what the spread and its own earlier passes wrote, so the runs repeat exactly)
+    N3Tuning cfg = tuning_for(pl, L, c, bstart, pp.bin_count, srt, svals, cg, std::min(L, lg_max));
+    last_tuning = cfg;
     for (int g0 = 0; g0 < L; g0 += lg_max) {
       const int lg = std::min(lg_max, L - g0);
       const int pass = n_passes++;
       auto mark = [&](int stage, int edge) {
         BIPP_HIP(hipEventRecord(pass_event(pass, stage, edge), st));
       };
-      // spread + deconvolution into [-nf1/2, nf1/2)^3 of the nf2 grid
-      const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
-      hdec.resize(nd);
-      for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
-      for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
-      for (int i = 0; i < pl.nf1[2]; ++i)
-        hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
-      TG* ddec = deconv.as<TG>(nd);
-      BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
       if (pass != first) {
         mark(0, 0);  // the points are sorted once per box
         mark(0, 1);
       }
       mark(1, 0);
-      N3SpreadParams<TG> sp;
-      sp.n_levels = lg;
-      sp.lv0 = g0;
-      sp.n_levels_total = L;
-      sp.lv_begin = g0;
-      sp.lv_end = g0 + lg;
-      sp.w = pl.w;
-      sp.beta = pl.beta;
-      for (int d = 0; d < 3; ++d) {
-        sp.bin_size[d] = pl.bin[d];
-        sp.nbins[d] = pl.nbins[d];
-        sp.nback[d] = pl.nback[d];
-        sp.nf1[d] = pl.nf1[d];
-        sp.nf2[d] = pl.nf2[d];
-      }
-      sp.bin_start = bstart;
-      sp.bin_count = pp.bin_count;
-      sp.pts = srt;
-      sp.vals = svals;
-      // blocks per XCD run of the spread: 32 (spread 9.4 ms at configuration 4 against 11.1 ms
-      // in launch order, 9.45 ms with 16 and 9.42 ms with 64); BIPP_SPREAD_XCD_CHUNK overrides
-      {
-        static const int chunk = [] {
-          const char* e = std::getenv("BIPP_SPREAD_XCD_CHUNK");
-          return e ? std::max(0, std::atoi(e)) : 32;
-        }();
-        sp.xcd_chunk = chunk;
-      }
-      sp.fx = ddec;
-      sp.fy = ddec + pl.nf1[0];
-      sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
-      sp.grid = cg;
-      sp.grid_stride = G;
-      // BIPP_N3_PROFILE=1: per-block duration / work summary of the spread on stderr
-      const bool prof = std::getenv("BIPP_N3_PROFILE") != nullptr;
-      sp.prof = prof ? prof_buf.as<unsigned long long>((size_t)nb * 4) : nullptr;
-      if (prof) BIPP_HIP(hipMemsetAsync(sp.prof, 0, sizeof(unsigned long long) * nb * 4, st));
-      launch_n3_spread<TG, cplx<TG>>(sp, st);
-      if (prof) report_spread_profile(sp.prof, nb, pl, st);
+      spread_group(pl, L, g0, lg, bstart, pp.bin_count, srt, svals, cg, cfg, st);
       mark(1, 1);
       mark(2, 0);
-
-      // pruned FFT (sign +) along x, y, z; twiddle tables exp(+2 pi i t / n) computed in FP64
-      const int nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
-      // table segments: x, then per strided axis n1, n2 and n1 * n2
-      size_t tw_off[3][3] = {};
-      htw.clear();
-      // two-level tables: exp(+2 pi i t / n) = fine[t mod 64] * coarse[t / 64]
-      auto add_table = [&](long long n) {
-        const size_t o = htw.size();
-        auto root = [&](long long t) {
-          const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t % n) / n;
-          return mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang)));
-        };
-        for (long long t = 0; t < 64; ++t) htw.push_back(root(t));
-        for (long long s = 0; s < (n + 63) / 64; ++s) htw.push_back(root(64 * s));
-        return o;
-      };
-      tw_off[0][0] = add_table(nx);
-      for (int d = 1; d < 3; ++d) {
-        tw_off[d][0] = add_table(pl.n1[d]);
-        tw_off[d][1] = add_table(pl.n2[d]);
-        tw_off[d][2] = pl.n2[d] > 1 ? add_table((long long)pl.n1[d] * pl.n2[d]) : 0;
-      }
-      cplx<TG>* dtw = twid.as<cplx<TG>>(htw.size());
-      BIPP_HIP(hipMemcpyAsync(dtw, htw.data(), sizeof(cplx<TG>) * htw.size(),
-                              hipMemcpyHostToDevice, st));
-      // x: lines (y, z) of the input window; read x of the input window, write the output window
-      N3LineParams<TG> lp;
-      lp.grid = cg;
-      lp.grid_stride = G;
-      lp.n_levels = lg;
-      lp.n = nx;
-      lp.ny = ny;
-      lp.nz = nz;
-      lp.yw = pl.in_w[1];
-      lp.zw = pl.in_w[2];
-      lp.in = pl.in_w[0];
-      lp.out = pl.out_w[0];
-      lp.rad = radices_of(nx, fft_radix_max());
-      lp.tw = dtw + tw_off[0][0];
-      launch_fft_lines<TG>(lp, st);
-      // y over (x of the output window, z of the input window); z over (x, y output windows)
-      for (int d = 1; d < 3; ++d) {
-        N3PassParams<TG> fp{};
-        fp.grid = cg;
-        fp.grid_stride = G;
-        fp.n_levels = lg;
-        fp.nx = nx;
-        fp.xw = pl.out_w[0];
-        if (d == 1) {
-          fp.no = nz;
-          fp.other = pl.in_w[2];
-          fp.operm_n1 = 1;
-          fp.operm_n2 = 1;
-          fp.outer_stride = (long long)nx * ny;
-          fp.axis_stride = nx;
-        } else {
-          fp.no = ny;
-          fp.other = pl.out_w[1];
-          fp.operm_n1 = pl.n1[1];
-          fp.operm_n2 = pl.n2[1];
-          fp.outer_stride = nx;
-          fp.axis_stride = (long long)nx * ny;
-        }
-        fp.nax = pl.nf2[d];
-        fp.in = pl.in_w[d];
-        fp.out = pl.out_w[d];
-        const int n1 = pl.n1[d], n2 = pl.n2[d];
-        if (n2 == 1) {
-          fp.m = n1;
-          fp.n_sub = 1;
-          fp.sub_stride = 0;
-          fp.estride = 1;
-          fp.in_check = 1;
-          fp.out_check = 1;
-          fp.fa = 0;
-          fp.fb = 1;
-          fp.twiddle = 0;
-          fp.ntot = n1;
-          fp.rad = radices_of(n1, fft_pass_radix_max());
-          fp.tw = dtw + tw_off[d][0];
-          fp.tw2 = nullptr;
-          launch_fft_pass<TG>(fp, st);
-        } else {
-          // n = n1 n2: transforms of length n1 over stride n2 (inputs pruned), twiddle, then
-          // transforms of length n2 over contiguous runs (outputs pruned); frequency k is
-          // stored at (k mod n1) n2 + k / n1
-          fp.m = n1;
-          fp.n_sub = n2;
-          fp.sub_stride = 1;
-          fp.estride = n2;
-          fp.in_check = 1;
-          fp.out_check = 0;
-          fp.twiddle = 1;
-          fp.ntot = (long long)n1 * n2;
-          fp.rad = radices_of(n1, fft_pass_radix_max());
-          fp.tw = dtw + tw_off[d][0];
-          fp.tw2 = dtw + tw_off[d][2];
-          launch_fft_pass<TG>(fp, st);
-          fp.m = n2;
-          fp.n_sub = n1;
-          fp.sub_stride = n2;
-          fp.estride = 1;
-          fp.in_check = 0;
-          fp.out_check = 1;
-          fp.fa = 1;
-          fp.fb = n1;
-          fp.twiddle = 0;
-          fp.rad = radices_of(n2, fft_pass_radix_max());
-          fp.tw = dtw + tw_off[d][1];
-          fp.tw2 = nullptr;
-          launch_fft_pass<TG>(fp, st);
-        }
-      }
-
+      fft_group(pl, lg, cg, cfg, st);
       mark(2, 1);
       mark(3, 0);
-      // interpolation, correction, real part into the images
-      double* dft = gl.as<double>(pl.ft_tab.size());
-      BIPP_HIP(hipMemcpyAsync(dft, pl.ft_tab.data(), sizeof(double) * pl.ft_tab.size(),
-                              hipMemcpyHostToDevice, st));
-      N3InterpParams<TG> ip;
-      ip.n_pix = n_pix;
-      ip.n_levels = lg;
-      ip.w1 = pl.w;
-      ip.w2 = pl.w;
-      ip.beta1 = pl.beta;
-      ip.beta2 = pl.beta;
-      for (int d = 0; d < 3; ++d) {
-        ip.gam[d] = pl.gam[d];
-        ip.D[d] = pl.D[d];
-        ip.C[d] = pl.C[d];
-        ip.h1[d] = pl.h1[d];
-        ip.h2[d] = pl.h2[d];
-        ip.nf2[d] = pl.nf2[d];
-        ip.n1[d] = pl.n1[d];
-        ip.n2[d] = pl.n2[d];
-      }
-      ip.lmn = lmn64;
-      ip.ft = dft;
-      ip.n_ft = static_cast<int>(pl.ft_tab.size());
-      ip.ft_inv_delta = pl.ft_inv_delta;
-      ip.grid = cg;
-      ip.grid_stride = G;
-      ip.img = static_cast<TG*>(img) + (size_t)g0 * n_pix;
-      ip.scale = 1.0;
-      ip.img_ld = n_pix;
-      launch_n3_interp<TG, TG>(ip, st);
+      interp_group(pl, g0, lg, cg, lmn64, n_pix, img, st);
       mark(3, 1);
     }
+  }
+
+  // host tables of the plan on the device: deconvolution factors, twiddles, kernel transform
+  void upload_tables(const N3Plan& pl, hipStream_t st) {
+    const int nd = pl.nf1[0] + pl.nf1[1] + pl.nf1[2];
+    hdec.resize(nd);
+    for (int i = 0; i < pl.nf1[0]; ++i) hdec[i] = static_cast<TG>(pl.deconv[0][i]);
+    for (int i = 0; i < pl.nf1[1]; ++i) hdec[pl.nf1[0] + i] = static_cast<TG>(pl.deconv[1][i]);
+    for (int i = 0; i < pl.nf1[2]; ++i)
+      hdec[pl.nf1[0] + pl.nf1[1] + i] = static_cast<TG>(pl.deconv[2][i]);
+    TG* ddec = deconv.as<TG>(nd);
+    BIPP_HIP(hipMemcpyAsync(ddec, hdec.data(), sizeof(TG) * nd, hipMemcpyHostToDevice, st));
+    // twiddle tables exp(+2 pi i t / n) computed in FP64: x, then per strided axis n1, n2 and
+    // n1 * n2; two-level tables: exp(+2 pi i t / n) = fine[t mod 64] * coarse[t / 64]
+    htw.clear();
+    auto add_table = [&](long long n) {
+      const size_t o = htw.size();
+      auto root = [&](long long t) {
+        const double ang = 2.0 * 3.14159265358979323846 * static_cast<double>(t % n) / n;
+        return mk<TG>(static_cast<TG>(std::cos(ang)), static_cast<TG>(std::sin(ang)));
+      };
+      for (long long t = 0; t < 64; ++t) htw.push_back(root(t));
+      for (long long s = 0; s < (n + 63) / 64; ++s) htw.push_back(root(64 * s));
+      return o;
+    };
+    tw_off[0][0] = add_table(pl.nf2[0]);
+    for (int d = 1; d < 3; ++d) {
+      tw_off[d][0] = add_table(pl.n1[d]);
+      tw_off[d][1] = add_table(pl.n2[d]);
+      tw_off[d][2] = pl.n2[d] > 1 ? add_table((long long)pl.n1[d] * pl.n2[d]) : 0;
+    }
+    cplx<TG>* dtw = twid.as<cplx<TG>>(htw.size());
+    BIPP_HIP(hipMemcpyAsync(dtw, htw.data(), sizeof(cplx<TG>) * htw.size(), hipMemcpyHostToDevice, st));
+    double* dft = gl.as<double>(pl.ft_tab.size());
+    BIPP_HIP(hipMemcpyAsync(dft, pl.ft_tab.data(), sizeof(double) * pl.ft_tab.size(),
+                            hipMemcpyHostToDevice, st));
+  }
+
+  // spread + deconvolution of levels [g0, g0 + lg) into [-nf1/2, nf1/2)^3 of the nf2 grid
+  void spread_group(const N3Plan& pl, int L, int g0, int lg, const unsigned* bstart,
+                    const unsigned* bcount, const N3Point* srt, const cplx<TG>* svals,
+                    cplx<TG>* cg, const N3Tuning& cfg, hipStream_t st) {
+    const TG* ddec = static_cast<const TG*>(deconv.p);
+    N3SpreadParams<TG> sp;
+    sp.n_levels = lg;
+    sp.lv0 = g0;
+    sp.n_levels_total = L;
+    sp.lv_begin = g0;
+    sp.lv_end = g0 + lg;
+    sp.w = pl.w;
+    sp.beta = pl.beta;
+    for (int d = 0; d < 3; ++d) {
+      sp.bin_size[d] = pl.bin[d];
+      sp.nbins[d] = pl.nbins[d];
+      sp.nback[d] = pl.nback[d];
+      sp.nf1[d] = pl.nf1[d];
+      sp.nf2[d] = pl.nf2[d];
+    }
+    sp.bin_start = bstart;
+    sp.bin_count = bcount;
+    sp.pts = srt;
+    sp.vals = svals;
+    sp.xcd_chunk = cfg.spread_chunk;
+    sp.fx = ddec;
+    sp.fy = ddec + pl.nf1[0];
+    sp.fz = ddec + pl.nf1[0] + pl.nf1[1];
+    sp.grid = cg;
+    sp.grid_stride = pl.grid_cells;
+    // BIPP_N3_PROFILE=1: per-block duration / work summary of the spread on stderr
+    const long long nb = pl.n_bins();
+    const bool prof = std::getenv("BIPP_N3_PROFILE") != nullptr;
+    sp.prof = prof ? prof_buf.as<unsigned long long>((size_t)nb * 4) : nullptr;
+    if (prof) BIPP_HIP(hipMemsetAsync(sp.prof, 0, sizeof(unsigned long long) * nb * 4, st));
+    launch_n3_spread<TG, cplx<TG>>(sp, st);
+    if (prof) report_spread_profile(sp.prof, nb, pl, st);
+  }
+
+  // pruned FFT (sign +) along x, y, z of lg level grids
+  void fft_group(const N3Plan& pl, int lg, cplx<TG>* cg, const N3Tuning& cfg, hipStream_t st) {
+    const long long G = pl.grid_cells;
+    const cplx<TG>* dtw = static_cast<const cplx<TG>*>(twid.p);
+    const int nx = pl.nf2[0], ny = pl.nf2[1], nz = pl.nf2[2];
+    // x: lines (y, z) of the input window; read x of the input window, write the output window
+    N3LineParams<TG> lp;
+    lp.grid = cg;
+    lp.grid_stride = G;
+    lp.n_levels = lg;
+    lp.n = nx;
+    lp.ny = ny;
+    lp.nz = nz;
+    lp.yw = pl.in_w[1];
+    lp.zw = pl.in_w[2];
+    lp.in = pl.in_w[0];
+    lp.out = pl.out_w[0];
+    lp.rad = radices_of(nx, cfg.lines_radix);
+    lp.tw = dtw + tw_off[0][0];
+    launch_fft_lines<TG>(lp, st);
+    // y over (x of the output window, z of the input window); z over (x, y output windows)
+    for (int d = 1; d < 3; ++d) {
+      N3PassParams<TG> fp{};
+      fp.grid = cg;
+      fp.grid_stride = G;
+      fp.n_levels = lg;
+      fp.nx = nx;
+      fp.xw = pl.out_w[0];
+      fp.tw_shift_max = cfg.pass_tw_shift;
+      if (d == 1) {
+        fp.no = nz;
+        fp.other = pl.in_w[2];
+        fp.operm_n1 = 1;
+        fp.operm_n2 = 1;
+        fp.outer_stride = (long long)nx * ny;
+        fp.axis_stride = nx;
+      } else {
+        fp.no = ny;
+        fp.other = pl.out_w[1];
+        fp.operm_n1 = pl.n1[1];
+        fp.operm_n2 = pl.n2[1];
+        fp.outer_stride = nx;
+        fp.axis_stride = (long long)nx * ny;
+      }
+      fp.nax = pl.nf2[d];
+      fp.in = pl.in_w[d];
+      fp.out = pl.out_w[d];
+      const int n1 = pl.n1[d], n2 = pl.n2[d];
+      if (n2 == 1) {
+        fp.m = n1;
+        fp.n_sub = 1;
+        fp.sub_stride = 0;
+        fp.estride = 1;
+        fp.in_check = 1;
+        fp.out_check = 1;
+        fp.fa = 0;
+        fp.fb = 1;
+        fp.twiddle = 0;
+        fp.ntot = n1;
+        fp.rad = radices_of(n1, cfg.pass_radix);
+        fp.tw = dtw + tw_off[d][0];
+        fp.tw2 = nullptr;
+        launch_fft_pass<TG>(fp, st);
+      } else {
+        // n = n1 n2: transforms of length n1 over stride n2 (inputs pruned), twiddle, then
+        // transforms of length n2 over contiguous runs (outputs pruned); frequency k is
+        // stored at (k mod n1) n2 + k / n1
+        fp.m = n1;
+        fp.n_sub = n2;
+        fp.sub_stride = 1;
+        fp.estride = n2;
+        fp.in_check = 1;
+        fp.out_check = 0;
+        fp.twiddle = 1;
+        fp.ntot = (long long)n1 * n2;
+        fp.rad = radices_of(n1, cfg.pass_radix);
+        fp.tw = dtw + tw_off[d][0];
+        fp.tw2 = dtw + tw_off[d][2];
+        launch_fft_pass<TG>(fp, st);
+        fp.m = n2;
+        fp.n_sub = n1;
+        fp.sub_stride = n2;
+        fp.estride = 1;
+        fp.in_check = 0;
+        fp.out_check = 1;
+        fp.fa = 1;
+        fp.fb = n1;
+        fp.twiddle = 0;
+        fp.rad = radices_of(n2, cfg.pass_radix);
+        fp.tw = dtw + tw_off[d][1];
+        fp.tw2 = nullptr;
+        launch_fft_pass<TG>(fp, st);
+      }
+    }
+  }
+
+  // interpolation, correction, real part into the images of levels [g0, g0 + lg)
+  void interp_group(const N3Plan& pl, int g0, int lg, const cplx<TG>* cg, const double* lmn64,
+                    int n_pix, void* img, hipStream_t st) {
+    N3InterpParams<TG> ip;
+    ip.n_pix = n_pix;
+    ip.n_levels = lg;
+    ip.w1 = pl.w;
+    ip.w2 = pl.w;
+    ip.beta1 = pl.beta;
+    ip.beta2 = pl.beta;
+    for (int d = 0; d < 3; ++d) {
+      ip.gam[d] = pl.gam[d];
+      ip.D[d] = pl.D[d];
+      ip.C[d] = pl.C[d];
+      ip.h1[d] = pl.h1[d];
+      ip.h2[d] = pl.h2[d];
+      ip.nf2[d] = pl.nf2[d];
+      ip.n1[d] = pl.n1[d];
+      ip.n2[d] = pl.n2[d];
+    }
+    ip.lmn = lmn64;
+    ip.ft = static_cast<const double*>(gl.p);
+    ip.n_ft = static_cast<int>(pl.ft_tab.size());
+    ip.ft_inv_delta = pl.ft_inv_delta;
+    ip.grid = cg;
+    ip.grid_stride = pl.grid_cells;
+    ip.img = static_cast<TG*>(img) + (size_t)g0 * n_pix;
+    ip.scale = 1.0;
+    ip.img_ld = n_pix;
+    launch_n3_interp<TG, TG>(ip, st);
+  }
+
+  // Run-time shapes for this grid shape: from the process-wide table of tuned shapes, or,
+  // with autotuning on and no entry yet, by timing every candidate (spread + FFT of one level
+  // group with HIP events) and keeping the fastest.
+  N3Tuning tuning_for(const N3Plan& pl, int L, ContextImpl& c, const unsigned* bstart,
+                      const unsigned* bcount, const N3Point* srt, const cplx<TG>* svals,
+                      cplx<TG>* cg, int lg) {
+    const std::string key = n3_tune_key(pl, L, sizeof(TG));
+    N3Tuning found;
+    if (n3_tune_lookup(key, found)) return found;
+    if (!autotune) return N3Tuning{};
+    hipStream_t st = c.stream;
+    const std::vector<N3Tuning> cands = n3_tune_candidates();
+    std::vector<hipEvent_t> ev(2 * cands.size());
+    for (auto& e : ev) BIPP_HIP(hipEventCreate(&e));
+    for (size_t k = 0; k < cands.size(); ++k) {
+      BIPP_HIP(hipEventRecord(ev[2 * k], st));
+      spread_group(pl, L, 0, lg, bstart, bcount, srt, svals, cg, cands[k], st);
+      fft_group(pl, lg, cg, cands[k], st);
+      BIPP_HIP(hipEventRecord(ev[2 * k + 1], st));
+    }
+    c.sync();
+    size_t best = 0;
+    float best_ms = 0;
+    tune_log.clear();
+    for (size_t k = 0; k < cands.size(); ++k) {
+      float t = 0;
+      BIPP_HIP(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+      tune_log.push_back({cands[k], t});
+      if (k == 0 || t < best_ms) {
+        best = k;
+        best_ms = t;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    n3_tune_store(key, cands[best]);
+    return cands[best];
   }
 };
 

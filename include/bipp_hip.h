@@ -183,6 +183,21 @@ enum { BIPP_PARTITION_AUTO = 0, BIPP_PARTITION_NONE = 1, BIPP_PARTITION_GRID = 2
 BippError bipp_synthesis_set_partition(BippSynthesis h, int method, const size_t* dims);
 /* Boxes per axis and boxes holding points of the last NUFFT collect. */
 BippError bipp_synthesis_nufft_partition(BippSynthesis h, int32_t* dims, int32_t* boxes);
+/* Autotuning of the type-3 NUFFT's run-time shapes (no counterpart in the reference, whose
+ * NUFFT comes from neonufft): with enable != 0, the first batch of every new grid shape times
+ * each candidate shape -- largest FFT radix of the x lines and of the strided passes, strided
+ * pass item width, spread blocks per XCD run -- on its own spread and FFT, and keeps the
+ * fastest for that shape in a process-wide table used by every later batch (and by other
+ * handles) whether or not they tune.  Off by default (measured defaults). */
+BippError bipp_synthesis_set_autotune(BippSynthesis h, int enable);
+/* Shape used by the last NUFFT batch, shape[4] = (lines radix, pass radix, log2 pass item
+ * width, spread chunk), and the candidates timed by the last tuning of this handle:
+ * 5 doubles each (the 4 shape values and milliseconds), up to capacity of them;
+ * *n_candidates receives their number (0 if this handle has not tuned). */
+BippError bipp_synthesis_nufft_tuning(BippSynthesis h, int32_t* shape, size_t capacity,
+                                      double* candidates, size_t* n_candidates);
+/* Forgets every tuned shape of the process. */
+BippError bipp_nufft_tuning_reset(void);
 /* Samples per batch (NufftSynthesisOptions::sampleBatchSize, src/nufft_synthesis.cpp:97-103);
  * 0: automatic (4096; the NUFFT imager min(4096, 2e9 / (20 nAntenna^2))). */
 BippError bipp_synthesis_set_sample_batch_size(BippSynthesis h, size_t n);

@@ -481,3 +481,35 @@ def test_nufft_memory_bound(ctx, cfg1_golden, monkeypatch, frac, boxed):
     for lvl in range(img.shape[0]):
         # two float32 images of the same float64 sums: a few 1e-8 apart each
         assert O.rel_l2(img[lvl], one[lvl]) < 3e-7
+
+
+def test_nufft_autotune(ctx, cfg1_golden):
+    """The autotuner times every candidate shape of the spread and FFT on the first batch of a
+    grid shape, keeps the fastest in the process-wide table (a later handle without tuning
+    uses it), and the image does not depend on the shape."""
+    g = cfg1_golden
+    bipp_amd.call("bipp_nufft_tuning_reset")
+    try:
+        ref = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                      imager="nufft", tolerance=1e-9)
+        ref.collect(g["wl"], g["s"], g["xyz"])
+        default_shape, none = ref.nufft_tuning()
+        assert none == []
+        syn = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                      imager="nufft", tolerance=1e-9, autotune=True)
+        syn.collect(g["wl"], g["s"], g["xyz"])
+        shape, rows = syn.nufft_tuning()
+        print("default", default_shape, "tuned", shape, flush=True)
+        for c, ms in rows:
+            print("  candidate", c, f"{ms:.3f} ms")
+        assert len(rows) == 24
+        assert shape == min(rows, key=lambda r: r[1])[0]
+        a, b = syn.images(), ref.images()
+        for lvl in range(a.shape[0]):
+            assert O.rel_l2(a[lvl], b[lvl]) < 3e-7
+        later = bipp_amd.NufftSynthesis(ctx, 37, 37, g["intervals"], g["lmn"], compute="fp64",
+                                        imager="nufft", tolerance=1e-9)
+        later.collect(g["wl"], g["s"], g["xyz"])
+        assert later.nufft_tuning() == (shape, [])
+    finally:
+        bipp_amd.call("bipp_nufft_tuning_reset")
