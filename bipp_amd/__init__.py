@@ -340,6 +340,21 @@ class NufftSynthesis:
             self._h, n_samples, wl_dev, PRECISION["double" if s_double else "single"], s_dev,
             nb, nb * nb, w_dev, na if w_dev else 0, na * nb if w_dev else 0, xyz_dev, na, 3 * na))
 
+    def collect_graph(self, n_samples: int, wl_dev: int, s_dev: int, s_double: bool,
+                      xyz_dev: int) -> None:
+        """Device arrays (identity beamforming, steering imager): the chain is captured in a
+        HIP graph on the first call for these pointers and replayed afterwards."""
+        nb, na = self.n_beam, self.n_antenna
+        check(lib().bipp_synthesis_collect_graph(
+            self._h, n_samples, wl_dev, PRECISION["double" if s_double else "single"], s_dev,
+            nb, nb * nb, xyz_dev, na, 3 * na))
+
+    def graph_info(self) -> Tuple[int, int]:
+        """(graph captures, graph launches) of collect_graph on this handle."""
+        a, b = C.c_longlong(), C.c_longlong()
+        check(lib().bipp_synthesis_graph_info(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def get(self, level: int, normalize: bool = True) -> np.ndarray:
         out = np.empty(self.n_pixel, dtype=np.float32)
         check(lib().bipp_synthesis_get(self._h, int(level), 1 if normalize else 0,

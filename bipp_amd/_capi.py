@@ -83,6 +83,8 @@ _SIGS = {
     "bipp_synthesis_destroy": (C.c_int, [vp]),
     "bipp_synthesis_collect": (C.c_int, [vp, sz, vp, C.c_int, vp, sz, sz, vp, sz, sz, vp, sz,
                                          sz]),
+    "bipp_synthesis_collect_graph": (C.c_int, [vp, sz, vp, C.c_int, vp, sz, sz, vp, sz, sz]),
+    "bipp_synthesis_graph_info": (C.c_int, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "bipp_synthesis_collect_host": (C.c_int, [vp, sz, dp, C.c_int, vp, sz, sz, vp, sz, sz, dp,
                                               sz, sz]),
     "bipp_synthesis_get": (C.c_int, [vp, sz, C.c_int, fp]),

@@ -305,6 +305,220 @@ struct BippSynthesisImpl {
     n_collected += nS;
   }
 
+  // ---- per-timestep chain captured in a HIP graph (steering imager, identity beamforming) ----
+  // flag -> eigensolve (samples in index order) -> level weights and operands -> steering
+  // image into a scratch image -> added to the images unless a solve failed.  No host
+  // round trip inside the chain, so it is captured once per argument set and replayed.
+  struct GraphState {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int nS = 0;
+    const double* wl = nullptr;
+    bool s_double = false;
+    const void* s = nullptr;
+    long long lds = 0, stride_s = 0;
+    const double* xyz = nullptr;
+    long long ldxyz = 0, stride_xyz = 0;
+    DevBuf n_red, idx, n_vis, info, counter, d, z, ws_a, ws_l, ws_z, n_eig, n_ant_used, pos,
+        afrag, wts, img_tmp, flag;
+    long long captures = 0, launches = 0;
+    void drop() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+    ~GraphState() { drop(); }
+  } gs;
+
+  template <typename R>
+  void record_chain(int nS, const double* wl, bool s_double, const void* s, long long lds,
+                    long long stride_s, const double* xyz, long long ldxyz, long long stride_xyz,
+                    int n_slots, hipStream_t st) {
+    const int nmax = n_beam;
+    const long long nn = (long long)nmax * nmax;
+    FlagParams fp;
+    fp.n_beam = n_beam;
+    fp.mode = 0;
+    fp.s = s;
+    fp.lds = lds;
+    fp.stride_s = stride_s;
+    fp.n_red = static_cast<int*>(gs.n_red.p);
+    fp.idx = static_cast<int*>(gs.idx.p);
+    fp.idx_stride = nmax;
+    fp.n_vis = static_cast<unsigned long long*>(gs.n_vis.p);
+    launch_flag(fp, nS, s_double, st);
+    EigParams<R> ep;
+    ep.n_jobs = nS;
+    ep.nmax = nmax;
+    ep.order = nullptr;
+    ep.counter = static_cast<unsigned*>(gs.counter.p);
+    launch_fill_u32(ep.counter, 0u, 1, st);
+    ep.n_red = fp.n_red;
+    ep.idx = fp.idx;
+    ep.idx_stride = nmax;
+    ep.s = s;
+    ep.lds = lds;
+    ep.stride_s = stride_s;
+    ep.xyz = xyz;
+    ep.ldxyz = ldxyz;
+    ep.stride_xyz = stride_xyz;
+    ep.wl = wl;
+    ep.stride_g = nn;
+    ep.ld_g = nmax;
+    ep.gram_prefactor = 1.0;
+    ep.standard = 0;
+    ep.d_out = static_cast<R*>(gs.d.p);
+    ep.stride_d = nmax;
+    ep.z_out = static_cast<cplx<R>*>(gs.z.p);
+    ep.stride_z = nn;
+    ep.ldz = nmax;
+    ep.ws_ld = nmax;
+    ep.ws_stride = nn;
+    ep.ws_a = static_cast<cplx<R>*>(gs.ws_a.p);
+    ep.ws_l = static_cast<cplx<R>*>(gs.ws_l.p);
+    ep.ws_z = static_cast<R*>(gs.ws_z.p);
+    ep.info = static_cast<int*>(gs.info.p);
+    launch_eig<R>(ep, n_slots, s_double, st);
+    const int mode = steer_mode();
+    const int apad = ((n_ant + 15) / 16) * 16;
+    const int kchunks = apad / 16;
+    const int rtiles = (2 * n_beam + 15) / 16;
+    const int epad = rtiles * 8;
+    const size_t fe = steer_frag_elems_per_sample(mode, rtiles, kchunks);
+    SteerPrepParams<R> pp;
+    pp.n_samples = nS;
+    pp.n_levels = n_levels;
+    pp.filter = filter;
+    pp.intervals = static_cast<const double*>(intervals.p);
+    pp.n_red = fp.n_red;
+    pp.idx = fp.idx;
+    pp.idx_stride = n_beam;
+    pp.d = ep.d_out;
+    pp.stride_d = n_beam;
+    pp.n_cols = n_beam;
+    pp.n_vis = fp.n_vis;
+    pp.z = ep.z_out;
+    pp.ldz = n_beam;
+    pp.stride_z = nn;
+    pp.xyz = xyz;
+    pp.ldxyz = ldxyz;
+    pp.stride_xyz = stride_xyz;
+    pp.wl = wl;
+    pp.n_eig = static_cast<int*>(gs.n_eig.p);
+    pp.n_ant = static_cast<int*>(gs.n_ant_used.p);
+    pp.pos_stride = 3LL * apad;
+    pp.pos = static_cast<double*>(gs.pos.p);
+    pp.apad = apad;
+    pp.afrag_stride = static_cast<long long>(fe);
+    pp.afrag = gs.afrag.p;
+    pp.kchunks = kchunks;
+    pp.rtiles = rtiles;
+    pp.w_stride = (long long)n_levels * epad;
+    pp.wts = gs.wts.p;
+    pp.epad = epad;
+    pp.mode = mode;
+    pp.n_rows_const = 0;
+    launch_steer_prep<R>(pp, st);
+    const long long n_img = (long long)n_levels * n_pix;
+    launch_zero_image(gs.img_tmp.p, n_img, img_double(), st);
+    SteerParams sp;
+    sp.n_samples = nS;
+    sp.n_levels = n_levels;
+    sp.n_pix = n_pix;
+    sp.n_eig = pp.n_eig;
+    sp.n_ant = pp.n_ant;
+    sp.pos = pp.pos;
+    sp.pos_stride = pp.pos_stride;
+    sp.apad = apad;
+    sp.afrag = pp.afrag;
+    sp.afrag_stride = pp.afrag_stride;
+    sp.kchunks = kchunks;
+    sp.rtiles = rtiles;
+    sp.wts = pp.wts;
+    sp.w_stride = pp.w_stride;
+    sp.epad = epad;
+    sp.lmn = static_cast<const double*>(lmn64.p);
+    sp.phase_fp32 = compute == BIPP_COMPUTE_FP32 ? 1 : 0;
+    sp.img = gs.img_tmp.p;
+    sp.img_ld = n_pix;
+    launch_steer(sp, mode, st);
+    launch_commit_image(ep.info, nS, gs.img_tmp.p, img.p, n_img, img_double(),
+                        static_cast<int*>(gs.flag.p), st);
+  }
+
+  template <typename R>
+  void collect_graph_impl(int nS, const double* wl, bool s_double, const void* s, long long lds,
+                          long long stride_s, const double* xyz, long long ldxyz,
+                          long long stride_xyz) {
+    ContextImpl& c = *ctx->impl;
+    hipStream_t st = c.stream;
+    const bool same = gs.exec && gs.nS == nS && gs.wl == wl && gs.s_double == s_double &&
+                      gs.s == s && gs.lds == lds && gs.stride_s == stride_s && gs.xyz == xyz &&
+                      gs.ldxyz == ldxyz && gs.stride_xyz == stride_xyz;
+    if (!same) {
+      gs.drop();
+      const int nmax = n_beam;
+      const long long nn = (long long)nmax * nmax;
+      const int per_cu = eig_blocks_per_cu<R>(nmax, s_double);
+      const int n_slots = std::max(1, std::min(nS, per_cu * c.n_cu));
+      const int mode = steer_mode();
+      const int apad = ((n_ant + 15) / 16) * 16;
+      const int rtiles = (2 * n_beam + 15) / 16;
+      const size_t fe = steer_frag_elems_per_sample(mode, rtiles, apad / 16);
+      const size_t esz = mode == STEER_BF16X3 ? 2 : (mode == STEER_FP32 ? 4 : 8);
+      // every buffer the graph touches is allocated before the capture and owned by it
+      gs.n_red.as<int>(nS);
+      gs.idx.as<int>((size_t)nS * nmax);
+      gs.n_vis.as<unsigned long long>(nS);
+      gs.info.as<int>(nS);
+      gs.counter.as<unsigned>(1);
+      gs.d.as<R>((size_t)nS * nmax);
+      gs.z.as<cplx<R>>((size_t)nS * nn);
+      gs.ws_a.as<cplx<R>>((size_t)n_slots * nn);
+      gs.ws_l.as<cplx<R>>((size_t)n_slots * nn);
+      gs.ws_z.as<R>((size_t)n_slots * nn);
+      gs.n_eig.as<int>(nS);
+      gs.n_ant_used.as<int>(nS);
+      gs.pos.as<double>((size_t)nS * 3 * apad);
+      gs.afrag.get((size_t)nS * fe * esz);
+      gs.wts.get((size_t)nS * n_levels * rtiles * 8 * (mode == STEER_FP64 ? 8 : 4));
+      gs.img_tmp.get((img_double() ? 8 : 4) * (size_t)n_levels * n_pix);
+      gs.flag.as<int>(1);
+      c.sync();
+      BIPP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        record_chain<R>(nS, wl, s_double, s, lds, stride_s, xyz, ldxyz, stride_xyz, n_slots, st);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      BIPP_HIP(hipStreamEndCapture(st, &gs.graph));
+      BIPP_HIP(hipGraphInstantiate(&gs.exec, gs.graph, nullptr, nullptr, 0));
+      gs.nS = nS;
+      gs.wl = wl;
+      gs.s_double = s_double;
+      gs.s = s;
+      gs.lds = lds;
+      gs.stride_s = stride_s;
+      gs.xyz = xyz;
+      gs.ldxyz = ldxyz;
+      gs.stride_xyz = stride_xyz;
+      ++gs.captures;
+    }
+    BIPP_HIP(hipGraphLaunch(gs.exec, st));
+    ++gs.launches;
+    int flag = 0;
+    BIPP_HIP(hipMemcpyAsync(&flag, gs.flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    c.sync();
+    if (flag)
+      fail(BIPP_EIGENSOLVER_ERROR,
+           "generalized eigensolve failed for a sample of the batch; the batch was not added");
+    n_collected += nS;
+  }
+
   void collect(int nS, const double* wl, bool s_double, const void* s, long long lds,
                long long stride_s, const void* w, long long ldw, long long stride_w,
                const double* xyz, long long ldxyz, long long stride_xyz) {
@@ -615,6 +829,38 @@ BippError bipp_synthesis_collect(BippSynthesis h, size_t nSamples, const double*
     if (nSamples == 0) return;
     h->collect((int)nSamples, wl, sPrec == BIPP_PRECISION_DOUBLE, s, lds, strideS, w, ldw, strideW,
                xyz, ldxyz, strideXyz);
+  });
+}
+
+BippError bipp_synthesis_collect_graph(BippSynthesis h, size_t nSamples, const double* wl,
+                                       BippPrecision sPrec, const void* s, size_t lds,
+                                       size_t strideS, const double* xyz, size_t ldxyz,
+                                       size_t strideXyz) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    require(s && xyz && wl, BIPP_INVALID_POINTER_ERROR, "null array");
+    require(h->imager == 0, BIPP_INVALID_PARAMETER_ERROR,
+            "graph-captured collects run the steering imager (the NUFFT plan depends on the data)");
+    require(h->n_ant == h->n_beam, BIPP_INVALID_PARAMETER_ERROR,
+            "graph-captured collects take identity beamforming (nAntenna == nBeam)");
+    require(lds >= (size_t)h->n_beam && ldxyz >= (size_t)h->n_ant, BIPP_INVALID_PARAMETER_ERROR,
+            "leading dimension too small");
+    require(nSamples <= 4096, BIPP_INVALID_PARAMETER_ERROR,
+            "a graph-captured collect takes at most 4096 samples");
+    if (nSamples == 0) return;
+    const bool sd = sPrec == BIPP_PRECISION_DOUBLE;
+    if (h->compute == BIPP_COMPUTE_FP32)
+      h->collect_graph_impl<float>((int)nSamples, wl, sd, s, lds, strideS, xyz, ldxyz, strideXyz);
+    else
+      h->collect_graph_impl<double>((int)nSamples, wl, sd, s, lds, strideS, xyz, ldxyz, strideXyz);
+  });
+}
+
+BippError bipp_synthesis_graph_info(BippSynthesis h, long long* captures, long long* launches) {
+  return guard([&] {
+    require(h != nullptr, BIPP_INVALID_HANDLE_ERROR, "null handle");
+    if (captures) *captures = h->gs.captures;
+    if (launches) *launches = h->gs.launches;
   });
 }
 

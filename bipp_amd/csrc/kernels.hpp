@@ -182,6 +182,10 @@ struct SteerParams {
 template <typename R>
 void launch_steer_prep(const SteerPrepParams<R>& p, hipStream_t stream);
 void launch_steer(const SteerParams& p, int mode, hipStream_t stream);
+// flag[0] = any info != 0; img += tmp (n elements, float or double) unless flagged
+void launch_commit_image(const int* info, int n_samples, const void* tmp, void* img, long long n,
+                         bool dbl, int* flag, hipStream_t stream);
+void launch_zero_image(void* p, long long n, bool dbl, hipStream_t stream);
 size_t steer_frag_elems_per_sample(int mode, int rtiles, int kchunks);
 
 template <typename R>

@@ -26,6 +26,8 @@
 // a 16-antenna chunk is generated once per workgroup into LDS (double-buffered) in the
 // exact MFMA B-operand layout.
 #include "common.hpp"
+
+#include <algorithm>
 #include "kernels.hpp"
 
 #include <type_traits>
@@ -528,6 +530,53 @@ void launch_steer(const SteerParams& p, int mode, hipStream_t stream) {
     steer_kernel<STEER_FP64><<<ceil_div(p.n_pix, TP), SteerCfg<STEER_FP64>::NT, 0, stream>>>(p);
   }
   check_launch("steer_kernel");
+}
+
+namespace {
+// flag[0] = 1 if any sample's solve failed (info != 0), else 0
+__global__ void any_failure_kernel(const int* info, int n, int* flag) {
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) bad |= info[i] != 0;
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) flag[0] = bad;
+}
+// img += tmp unless flag[0] is set
+template <typename T>
+__global__ void commit_image_kernel(const T* tmp, T* img, long long n, const int* flag) {
+  if (flag[0]) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    img[i] += tmp[i];
+}
+template <typename T>
+__global__ void zero_image_kernel(T* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = T(0);
+}
+}  // namespace
+
+void launch_commit_image(const int* info, int n_samples, const void* tmp, void* img, long long n,
+                         bool dbl, int* flag, hipStream_t stream) {
+  any_failure_kernel<<<1, 256, 0, stream>>>(info, n_samples, flag);
+  check_launch("any_failure_kernel");
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 4096));
+  if (dbl)
+    commit_image_kernel<double><<<blocks, 256, 0, stream>>>(static_cast<const double*>(tmp),
+                                                           static_cast<double*>(img), n, flag);
+  else
+    commit_image_kernel<float><<<blocks, 256, 0, stream>>>(static_cast<const float*>(tmp),
+                                                          static_cast<float*>(img), n, flag);
+  check_launch("commit_image_kernel");
+}
+
+void launch_zero_image(void* p, long long n, bool dbl, hipStream_t stream) {
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 4096));
+  if (dbl)
+    zero_image_kernel<double><<<blocks, 256, 0, stream>>>(static_cast<double*>(p), n);
+  else
+    zero_image_kernel<float><<<blocks, 256, 0, stream>>>(static_cast<float*>(p), n);
+  check_launch("zero_image_kernel");
 }
 
 template <typename R>

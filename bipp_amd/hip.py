@@ -68,6 +68,14 @@ class DeviceArray:
         _ok(hip().hipMemset(d.ptr, 0, max(d.nbytes, 1)), "hipMemset")
         return d
 
+    def upload(self, a: np.ndarray) -> None:
+        """Copy host data of the same dtype and byte size into this allocation."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("upload size differs from the allocation")
+        synchronize()
+        _ok(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         synchronize()

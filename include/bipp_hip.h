@@ -138,7 +138,20 @@ BippError bipp_synthesis_collect(BippSynthesis h, size_t nSamples, const double*
                                  BippPrecision sPrec, const void* s, size_t lds, size_t strideS,
                                  const void* w, size_t ldw, size_t strideW, const double* xyz,
                                  size_t ldxyz, size_t strideXyz);
-/* Same with host arrays (copied to the device first). */
+/* Per-timestep chain captured in a HIP graph: the same as bipp_synthesis_collect for the
+ * steering imager with identity beamforming (w = NULL), with no host round trip inside the
+ * chain (samples solved in index order; a failed solve keeps the whole batch out of the
+ * images and returns BIPP_EIGENSOLVER_ERROR).  The first call for an argument set captures
+ * flag -> eigensolve -> level weights -> steering image into a graph; later calls with the
+ * same pointers, sizes and strides replay it, so a caller refills the same device buffers
+ * for every timestep.  No counterpart in the reference. */
+BippError bipp_synthesis_collect_graph(BippSynthesis h, size_t nSamples, const double* wl,
+                                       BippPrecision sPrec, const void* s, size_t lds,
+                                       size_t strideS, const double* xyz, size_t ldxyz,
+                                       size_t strideXyz);
+/* Graph captures and launches of bipp_synthesis_collect_graph on this handle. */
+BippError bipp_synthesis_graph_info(BippSynthesis h, long long* captures, long long* launches);
+/* Same as bipp_synthesis_collect with host arrays (copied to the device first). */
 BippError bipp_synthesis_collect_host(BippSynthesis h, size_t nSamples, const double* wl,
                                       BippPrecision sPrec, const void* s, size_t lds,
                                       size_t strideS, const void* w, size_t ldw, size_t strideW,
