@@ -3,9 +3,11 @@
     python -m bipp_amd.traffic --fetch DIR --write DIR --kernel eig_kernel \\
         --copy-bytes N --command "..." --out profiles/eig_kernel_traffic.json
 
-Each run's counter CSV holds, per dispatch, the counter in KiB.  The runs also hold one
-device-to-device copy of N bytes (the runtime's copy kernel, which reads N and writes N
-bytes); the counter it shows for that copy gives the scale factor applied to the kernel's.
+Each run's counter CSV holds, per dispatch, the counter in KiB.  The runs also hold
+device-to-device copies of N bytes (the runtime's copy kernel, which reads N and writes N
+bytes); the counter shown for such a copy gives the scale factor applied to the kernel's.
+When a run holds two or more of them (bench.py --copy-calibration puts one on each side of
+its timed passes), only the kernel dispatches between the first and the last are averaged.
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ from collections import defaultdict
 
 
 def per_dispatch(run_dir: str, counter: str):
+    """(file, dispatch id, kernel name, counter summed over its instances) per dispatch."""
     files = glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no counter_collection.csv under {run_dir}")
@@ -28,21 +31,33 @@ def per_dispatch(run_dir: str, counter: str):
             for row in csv.DictReader(fh):
                 if row.get("Counter_Name") != counter:
                     continue
-                key = (fn, row["Dispatch_Id"])
+                key = (fn, int(row["Dispatch_Id"]))
                 vals[key] += float(row["Counter_Value"])
                 names[key] = row["Kernel_Name"]
-    return [(names[k], v) for k, v in vals.items()]
+    return [(k[0], k[1], names[k], v) for k, v in sorted(vals.items())]
 
 
-def mean_for(rows, needle: str):
-    sel = [v for n, v in rows if needle in n]
-    return (sum(sel) / len(sel), len(sel)) if sel else (None, 0)
+def calibration_copies(rows, needle: str):
+    """The calibration copies: dispatches of the copy kernel within a factor of two of the
+    largest one (every other copy of the program is far smaller)."""
+    sel = [r for r in rows if needle in r[2]]
+    if not sel:
+        return []
+    top = max(r[3] for r in sel)
+    return [r for r in sel if r[3] >= 0.5 * top]
 
 
-def largest_for(rows, needle: str):
-    """The calibration copy is by far the largest dispatch of the copy kernel."""
-    sel = [v for n, v in rows if needle in n]
-    return max(sel) if sel else None
+def window_mean(rows, needle: str, copies):
+    """Mean counter of the kernel's dispatches, restricted to those between the first and the
+    last calibration copy of the same file when there are two or more."""
+    bounds = {}
+    for fn, did, _, _ in copies:
+        lo, hi = bounds.get(fn, (did, did))
+        bounds[fn] = (min(lo, did), max(hi, did))
+    windowed = len(copies) >= 2
+    sel = [v for fn, did, n, v in rows if needle in n and
+           (not windowed or (fn in bounds and bounds[fn][0] < did < bounds[fn][1]))]
+    return (sum(sel) / len(sel), len(sel), windowed) if sel else (None, 0, windowed)
 
 
 def main() -> None:
@@ -57,17 +72,20 @@ def main() -> None:
     a = ap.parse_args()
     f_rows = per_dispatch(a.fetch, "FETCH_SIZE")
     w_rows = per_dispatch(a.write, "WRITE_SIZE")
-    fk, nf = mean_for(f_rows, a.kernel)
-    wk, nw = mean_for(w_rows, a.kernel)
-    fc = largest_for(f_rows, a.copy_kernel)
-    wc = largest_for(w_rows, a.copy_kernel)
-    if fk is None or wk is None or not fc or not wc:
+    f_cp = calibration_copies(f_rows, a.copy_kernel)
+    w_cp = calibration_copies(w_rows, a.copy_kernel)
+    fk, nf, f_win = window_mean(f_rows, a.kernel, f_cp)
+    wk, nw, w_win = window_mean(w_rows, a.kernel, w_cp)
+    if fk is None or wk is None or not f_cp or not w_cp:
         raise SystemExit("kernel or copy dispatches missing from the counter files")
+    fc = sum(r[3] for r in f_cp) / len(f_cp)
+    wc = sum(r[3] for r in w_cp) / len(w_cp)
     expect_kib = a.copy_bytes / 1024.0
     f_scale, w_scale = fc / expect_kib, wc / expect_kib
     out = {
         "kernel": a.kernel,
         "dispatches": {"fetch": nf, "write": nw},
+        "timed_window_only": bool(f_win and w_win),
         "fetch_kib_per_dispatch": fk,
         "write_kib_per_dispatch": wk,
         "copy_calibration": {"bytes": a.copy_bytes, "fetch_kib": fc, "write_kib": wc,
