@@ -365,7 +365,7 @@ def main() -> None:
                              "kernel": "steer_kernel<BF16X3>", "kernel_ms": kern_ms,
                              "algorithmic_flops_per_launch": flops,
                              "mfma_flops_executed_per_launch": 3 * flops}
-        img_ms = kern_ms
+        img_ms = kern_ms  # one launch
     else:
         # type-3 NUFFT chain (sort + spread + FFT + interpolation), algorithmic bytes: every
         # point's coordinates (12 B) once and its value (8 B) per level, the spread grid
@@ -404,13 +404,15 @@ def main() -> None:
             stage_rl["nufft"]["traffic_over_algorithmic"] = tot / nufft_bytes
             stage_rl["nufft"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                                                    "summaries in profiles/")
-        img_ms = stages["image_ms"]
+        # the chain's longest stage (its kernels run one after another)
+        img_ms = max(stages[k] for k in ("nufft_sort_ms", "nufft_spread_ms", "nufft_fft_ms",
+                                         "nufft_interp_ms"))
     # the dominant kernel of the step is reported as "roofline"
     if eig_ms >= img_ms or not stage_rl:
         roofline = rl_eig
     else:
         key = next(iter(stage_rl))
-        roofline = dict(stage_rl.pop(key), traffic=None)
+        roofline = stage_rl.pop(key)
         stage_rl["eigensolve"] = rl_eig
     cpu = None
     if not a.no_cpu:

@@ -169,26 +169,19 @@ struct EigRunner {
       BIPP_HIP(hipMemcpyAsync(h.data(), ep.prof, sizeof(unsigned long long) * h.size(),
                               hipMemcpyDeviceToHost, st));
       BIPP_HIP(hipStreamSynchronize(st));
-      static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "ql",
+      static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "tridiag_eig",
                                      "backtransform", "unwhiten"};
-      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, gen = 0, sweeps = 0, rots = 0;
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0};
       int cnt = 0;
       for (int k = 0; k < nS; ++k) {
         const unsigned long long* r = &h[(size_t)k * 16];
         if (!r[0] || !r[7]) continue;
         ++cnt;
         for (int q = 0; q < 7; ++q) acc[q] += r[q + 1] >= r[q] && r[q] ? double(r[q + 1] - r[q]) : 0.0;
-        gen += double(r[8]);
-        sweeps += double(r[9]);
-        rots += double(r[10]);
       }
       std::fprintf(stderr, "[bipp eig profile] %d samples, n_slots %d, us/sample (100 MHz clock):", cnt,
                    n_slots);
       for (int q = 0; q < 7; ++q) std::fprintf(stderr, " %s %.1f", names[q], cnt ? acc[q] / cnt / 100.0 : 0.0);
-      // time inside the rotation generator, and QL sweeps and rotations per sample
-      if (cnt)
-        std::fprintf(stderr, " | ql generator %.1f, sweeps %.0f, rotations %.0f", gen / cnt / 100.0,
-                     sweeps / cnt, rots / cnt);
       std::fprintf(stderr, "\n");
     }
 

@@ -3,19 +3,23 @@
 // Restates, per sample, src/eigensolver.cpp:129-225 (flag -> reduce -> Gram -> ?hegv)
 // with the LAPACK steps of ?hegv(itype=1,'V','L') (src/host/lapack_api.hpp:112-158):
 //   Cholesky G = L L^H (left-looking), W = L^-1, C = W S W^H (two LDS-tiled GEMMs),
-//   Householder tridiagonalisation C = Q T Q^H on the lower triangle, implicit QL on T
-//   (eigenvalues + rotations into Z, sweeps generated and applied in pairs), X = Q Z
-//   (eigenvector columns in registers), eigenvectors W^H X (GEMM), eigenvalues ascending.
+//   Householder tridiagonalisation C = Q T Q^H on the lower triangle, eigenpairs of T by
+//   divide and conquer (tridiag_dc.hpp, eigenvalues ascending, eigenvectors into Z), X = Q Z
+//   (eigenvector columns in registers), eigenvectors W^H X (GEMM).  ?hegv's QL iteration on T
+//   gives the same eigenpairs to rounding (eigenvectors of repeated eigenvalues up to a
+//   rotation within their eigenspace).
 //
 // Launch shape: a persistent grid (512-thread workgroups for problems above 128, 256
 // below); each workgroup pulls sample indices from a device counter (largest problems
 // first) and solves one problem at a time, keeping its matrices in a per-workgroup slot of
-// global memory and its vectors, tiles and rotation buffers in LDS.  The column steps are
+// global memory and its vectors, tiles and per-merge arrays in LDS.  The column steps are
 // workgroup-wide parallel updates separated by barriers, with lanes mapped to matrix rows
-// so that global accesses are coalesced; the QL rotation chain runs on one lane of a
-// dedicated wavefront.
+// so that global accesses are coalesced.
 #include "common.hpp"
 #include "kernels.hpp"
+
+#define BIPP_HD __device__
+#include "tridiag_dc.hpp"
 
 #include <limits>
 
@@ -66,13 +70,6 @@ __device__ __forceinline__ cplx<R> crecip(cplx<R> a) {
   }
 }
 
-// State of the tridiagonal QL iteration, advanced by one generator thread.
-template <typename R>
-struct QlState {
-  R f, tst1;
-  int l, m, iter, in_iter, fail;
-};
-
 // In-workgroup GEMM tiling: GT x GT output tiles, k taken GK at a time through LDS.
 constexpr int GT = 64;
 constexpr int GK = 16;
@@ -108,25 +105,30 @@ struct alignas(2 * sizeof(R)) RPair {
 template <typename R, int NMAX>
 struct EigShared {
   static constexpr int NW_ = EigCfg<NMAX>::NW;
-  cplx<R> v[NMAX];
-  cplx<R> w[NMAX];
-  cplx<R> vn[NMAX];
-  cplx<R> xc[NMAX];  // column (conjugate-transpose) parts of the Hermitian products
+  union {
+    struct {
+      cplx<R> v[NMAX];
+      cplx<R> w[NMAX];
+      cplx<R> vn[NMAX];
+      cplx<R> xc[NMAX];  // column (conjugate-transpose) parts of the Hermitian products
+    };
+    R dct[DcCfg<NMAX>::TILE_ELEMS];  // product tiles of the tridiagonal divide and conquer
+  };
   union {
     cplx<R> xw[NW_][NMAX];  // per-wave partial sums of the matrix-vector passes
     GemmTiles<R> gt;        // operand tiles of the GEMM passes
     BackTile<R, NMAX> bt;   // reflectors of the back-transform
+    DcShared<R, NMAX> dc;   // per-merge arrays of the tridiagonal divide and conquer
   };
   RPair<R> de[NMAX];      // tridiagonal: (diagonal d_i, subdiagonal e_i)
-  RPair<R> rot[2][2][NMAX];  // rotations (c_i, s_i) of two pairs of QL sweeps
   int perm[NMAX];
   int idx[NMAX];
   R red[2 * NW_];
   R scal[4];
   int iscal[4];
-  int sweep[2][2][3];  // (l, m, valid) of the rotation buffers
-  QlState<R> ql;
 };
+static_assert(DcCfg<256>::TILE_ELEMS <= 8 * 256 && DcCfg<64>::TILE_ELEMS <= 8 * 64,
+              "divide-and-conquer tiles fit in the four vectors");
 
 // --------------------------------------------------------------------------------------
 // W = L^-1 in place on the lower triangle of L (ztrti2, lower, non-unit): columns from the
@@ -440,238 +442,6 @@ __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<
   __syncthreads();
 }
 
-// g = sqrt(x) and h = 1/(2 sqrt(x)) for normal x > 0 from the hardware reciprocal square
-// root (about 2^-25 relative) and coupled Newton steps (each squares the relative error;
-// two steps reach 1 ulp in double), so that the rotation chain holds no correctly rounded
-// square root or division.
-__device__ __forceinline__ void sqrt_half_rsqrt(double x, double& g, double& h) {
-  const double y = __builtin_amdgcn_rsq(x);
-  g = x * y;
-  h = 0.5 * y;
-  double t = fma(-g, h, 0.5);
-  g = fma(g, t, g);
-  h = fma(h, t, h);
-  t = fma(-g, h, 0.5);
-  g = fma(g, t, g);
-  h = fma(h, t, h);
-}
-__device__ __forceinline__ void sqrt_half_rsqrt(float x, float& g, float& h) {
-  const float y = __builtin_amdgcn_rsqf(x);
-  g = x * y;
-  h = 0.5f * y;
-  const float t = fmaf(-g, h, 0.5f);
-  g = fmaf(g, t, g);
-  h = fmaf(h, t, h);
-}
-
-template <typename R>
-__device__ __forceinline__ R min_normal();
-template <>
-__device__ __forceinline__ double min_normal<double>() { return 2.2250738585072014e-308; }
-template <>
-__device__ __forceinline__ float min_normal<float>() { return 1.17549435e-38f; }
-
-// Bounds of max |d_i| + |e_i| and min |e_i| over a sweep within which the rotation chain
-// needs no guard: pp^2 + e_i^2 stays normal and finite.
-template <typename R>
-__device__ __forceinline__ R chain_lo();
-template <>
-__device__ __forceinline__ double chain_lo<double>() { return 1e-150; }
-template <>
-__device__ __forceinline__ float chain_lo<float>() { return 1e-18f; }
-template <typename R>
-__device__ __forceinline__ R chain_hi();
-template <>
-__device__ __forceinline__ double chain_hi<double>() { return 1e150; }
-template <>
-__device__ __forceinline__ float chain_hi<float>() { return 1e17f; }
-
-// One step of a QL sweep at index i: the Givens rotation zeroing e_i, r = sqrt(pp^2 + e_i^2),
-// c = pp / r, s = e_i / r, and the next pp = c d_i - s g = (pp d_i - e_i g) / r, whose
-// numerator is formed while the square root is refined (the dependency chain runs through
-// r only).  Writes (d_{i+1}, e_{i+1}) and (c, s) with one LDS access each.  GUARD: r = 0
-// (pp = e_i = 0, or pp^2 + e_i^2 below the normal range) is the identity rotation.
-template <typename R, bool GUARD>
-__device__ __forceinline__ void ql_step(int i, R ei, R di, R& pp, R& c, R& s, RPair<R>* de,
-                                        RPair<R>* rot) {
-  const R g = c * ei;
-  const R h = c * pp;
-  const R x = fma(pp, pp, ei * ei);
-  const R wnum = fma(pp, di, -(ei * g));
-  R rr, hr;
-  sqrt_half_rsqrt(x, rr, hr);
-  const R ir = hr + hr;  // 1/r (doubling is exact)
-  R enew;
-  if (GUARD) {
-    const bool ok = x >= min_normal<R>();
-    enew = ok ? s * rr : R(0);
-    s = ok ? ei * ir : R(0);
-    c = ok ? pp * ir : R(1);
-    pp = ok ? wnum * ir : di;
-  } else {
-    enew = s * rr;
-    s = ei * ir;
-    c = pp * ir;
-    pp = wnum * ir;
-  }
-  RPair<R> o;
-  o.x = h + s * (c * g + s * di);
-  o.y = enew;
-  de[i + 1] = o;
-  RPair<R> cs;
-  cs.x = c;
-  cs.y = s;
-  rot[i] = cs;
-}
-
-// Rotations i = m-1 .. l of one sweep on one lane, two steps per iteration with the operands
-// of the next two steps loaded ahead (they are not written by this sweep before use).
-template <typename R, bool GUARD>
-__device__ __forceinline__ void ql_chain(int l, int m, R& pp, R& c, R& s, RPair<R>* de,
-                                         RPair<R>* rot) {
-  RPair<R> cur = de[m - 1];
-  int i = m - 1;
-  for (; i - 1 >= l; i -= 2) {
-    const RPair<R> p1 = de[i - 1];
-    const RPair<R> p2 = de[i - 2 >= l ? i - 2 : i - 1];
-    ql_step<R, GUARD>(i, cur.y, cur.x, pp, c, s, de, rot);
-    ql_step<R, GUARD>(i - 1, p1.y, p1.x, pp, c, s, de, rot);
-    cur = p2;
-  }
-  if (i >= l) ql_step<R, GUARD>(i, cur.y, cur.x, pp, c, s, de, rot);
-}
-
-// Advances the tql2 iteration (EISPACK/JAMA implicit QL with Wilkinson-type shift) to its
-// next sweep and stores that sweep's rotations i = m-1 .. l in rot_out and (l, m) in sw;
-// returns false
-// when every eigenvalue has converged (or the iteration limit was hit, ql.fail = 1).
-// Runs on all 64 lanes of one wavefront: the state is uniform, the deflation scan, the
-// shift of d and the range check of the sweep are split over the lanes, and the rotation
-// chain runs on lane 0.  d, e and the state live in LDS.
-template <typename R, int NMAX>
-__device__ bool ql_next_sweep(int n, EigShared<R, NMAX>& sh, RPair<R>* rot_out, int* sw,
-                              int lane, unsigned long long& n_rot) {
-  QlState<R>& q = sh.ql;
-  RPair<R>* de = sh.de;
-  const R eps = eps_r<R>();
-  int l = q.l, m = q.m, iter = q.iter, in_iter = q.in_iter;
-  R f = q.f, tst1 = q.tst1;
-  bool more = false;
-  while (l < n) {
-    if (!in_iter) {
-      tst1 = fmax(tst1, fabs(de[l].x) + fabs(de[l].y));
-      // m = first index >= l with |e[m]| <= eps tst1 (NaN stops the scan), else n - 1
-      int mm = n - 1;
-      for (int b = l; b < n - 1; b += 64) {
-        const int k = b + lane;
-        const bool small = k < n - 1 && !(fabs(de[k].y) > eps * tst1);
-        const unsigned long long bal = __ballot(small);
-        if (bal) {
-          mm = b + __ffsll(static_cast<long long>(bal)) - 1;
-          break;
-        }
-      }
-      m = mm;
-      if (m == l) {
-        if (lane == 0) {
-          de[l].x += f;
-          de[l].y = 0;
-        }
-        wave_sync();
-        ++l;
-        continue;
-      }
-      in_iter = 1;
-      iter = 0;
-    }
-    if (++iter > 30 * n) {
-      if (lane == 0) q.fail = 1;
-      l = n;
-      break;
-    }
-    // shift from the leading 2x2 block (uniform over the lanes)
-    const R g0 = de[l].x;
-    const R el = de[l].y;
-    R pp = (de[l + 1].x - g0) / (R(2) * el);
-    R r = sqrt(pp * pp + R(1));
-    if (pp < 0) r = -r;
-    const R dl = el / (pp + r);
-    const R dl1 = el * (pp + r);
-    const R h0 = g0 - dl;
-    // shift d[l+2:n) and bound the sweep's operands: max |d_i| + |e_i| over [l, m] and
-    // min |e_i| over [l, m) (NaN fails the check below)
-    R big = 0, small = std::numeric_limits<R>::infinity();
-    for (int i = l + lane; i < n; i += 64) {
-      RPair<R> v = de[i];
-      if (i >= l + 2) {
-        v.x -= h0;
-        de[i].x = v.x;
-      } else {
-        v.x = i == l ? dl : dl1;
-      }
-      if (i <= m) big = fmax(big, fabs(v.x) + fabs(v.y));
-      if (i < m) small = fmin(small, fabs(v.y));
-      if (!(v.x == v.x) || !(v.y == v.y)) big = std::numeric_limits<R>::infinity();
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      big = fmax(big, __shfl_xor(big, off, 64));
-      small = fmin(small, __shfl_xor(small, off, 64));
-    }
-    const bool fast = big <= chain_hi<R>() && small >= chain_lo<R>();
-    f += h0;
-    wave_sync();  // lane 0 reads the shifted d below
-    int conv = 0;
-    if (lane == 0) {
-      de[l].x = dl;
-      de[l + 1].x = dl1;
-      pp = de[m].x;
-      R c = 1, s = 0;
-      const R el1 = de[l + 1].y;
-      RPair<R>* rot = rot_out;
-      if (fast)
-        ql_chain<R, false>(l, m, pp, c, s, de, rot);
-      else
-        ql_chain<R, true>(l, m, pp, c, s, de, rot);
-      // s and c of steps l+1 and l+2 (EISPACK's s2 and c3), 0 and 1 where absent
-      const R s2 = m - 1 >= l + 1 ? rot[l + 1].y : R(0);
-      const R c3 = m - 1 >= l + 2 ? rot[l + 2].x : R(1);
-      pp = -s * s2 * c3 * el1 * el / dl1;
-      RPair<R> o;
-      o.x = c * pp;
-      o.y = s * pp;
-      sw[0] = l;
-      sw[1] = m;
-      if (!(fabs(o.y) > eps * tst1)) {
-        // converged: this sweep is the last one for l
-        o.x += f;
-        o.y = 0;
-        conv = 1;
-      }
-      de[l] = o;
-      n_rot += static_cast<unsigned long long>(m - l);
-    }
-    conv = __shfl(conv, 0);
-    wave_sync();
-    if (conv) {
-      ++l;
-      in_iter = 0;
-    }
-    more = true;
-    break;
-  }
-  if (lane == 0) {
-    q.l = l;
-    q.m = m;
-    q.iter = iter;
-    q.in_iter = in_iter;
-    q.f = f;
-    q.tst1 = tst1;
-  }
-  wave_sync();
-  return more;
-}
-
 // records the wall clock at phase boundaries when the caller asked for a profile
 template <typename R>
 __device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
@@ -880,122 +650,17 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   }
   eig_mark(p, sid, 4);
 
-  // ---- implicit QL on the real tridiagonal (tql2), rotations into Z -------------------
-  // The last wavefront produces the rotations of the next pair of sweeps while the other
-  // threads apply the current pair to their rows of Z (double-buffered pairs).
-  for (int j = wave; j < n; j += NW_)
-    for (int i = lane; i < n; i += 64) Z[i + (size_t)j * ld] = (i == j) ? R(1) : R(0);
-  static_assert(NW_ >= 2, "a generator wavefront besides the row owners");
-  const int gwave = NW_ - 1;
-  // wavefronts are placed on the 4 SIMDs in turn: with 8 of them, wave gwave - 4 shares the
-  // generator's SIMD and owns no rows, so that the chain has its SIMD's issue slots
-  const int sib = NW_ >= 8 ? gwave - 4 : -1;
-  constexpr int NRT = NT_ - (NW_ >= 8 ? 128 : 64);  // row-owner threads
-  const int rrank = wave < sib || sib < 0 ? tid : tid - 64;
-  unsigned long long gen_cycles = 0, n_rot = 0, n_sweep = 0;
-  // up to two sweeps into pair buffer pb (the second only while the iteration goes on)
-  auto gen_pair = [&](int pb) {
-    const unsigned long long t0 = p.prof ? wall_clock64() : 0;
-    const bool more = ql_next_sweep<R, NMAX>(n, sh, sh.rot[pb][0], sh.sweep[pb][0], lane, n_rot);
-    const bool more2 =
-        more ? ql_next_sweep<R, NMAX>(n, sh, sh.rot[pb][1], sh.sweep[pb][1], lane, n_rot) : false;
-    if (lane == 0) {
-      sh.sweep[pb][0][2] = more ? 1 : 0;
-      sh.sweep[pb][1][2] = more2 ? 1 : 0;
-    }
-    if (p.prof) gen_cycles += wall_clock64() - t0;
-  };
-  if (wave == gwave) {
-    if (lane == 0) {
-      sh.ql.f = 0;
-      sh.ql.tst1 = 0;
-      sh.ql.l = 0;
-      sh.ql.m = 0;
-      sh.ql.iter = 0;
-      sh.ql.in_iter = 0;
-      sh.ql.fail = 0;
-    }
-    wave_sync();
-    gen_pair(0);
-  }
-  __syncthreads();
-  for (int t = 0;; ++t) {
-    const int cur = t & 1;
-    if (!sh.sweep[cur][0][2]) break;
-    const int lA = sh.sweep[cur][0][0], mA = sh.sweep[cur][0][1];
-    const bool vB = sh.sweep[cur][1][2] != 0;
-    const int lB = vB ? sh.sweep[cur][1][0] : 0, mB = vB ? sh.sweep[cur][1][1] : 0;
-    n_sweep += vB ? 2 : 1;
-    if (wave == gwave) gen_pair(cur ^ 1);
-    // a sweep always satisfies 0 <= l < m < n; anything else is skipped, never indexed
-    const bool run = lA >= 0 && lA < mA && mA < n && (!vB || (lB >= 0 && lB < mB && mB < n));
-    const int top = vB ? max(mA, mB) : mA, bot = vB ? min(lA, lB) : lA;
-    const unsigned spanA = static_cast<unsigned>(mA - lA);
-    const unsigned spanB = vB ? static_cast<unsigned>(mB - lB) : 0u;
-    // Each row passes once over columns top .. bot: sweep A's rotation at i (identity
-    // outside [lA, mA)) and, one column behind, sweep B's at i + 1 on A's result (identity
-    // outside [lB, mB)), so a pair of sweeps loads and stores each element once.  Columns
-    // are fetched CH at a time, the next chunk's loads issued before the current chunk is
-    // rotated and stored (stores land at columns above every pending load).
-    constexpr int CH = 24;
-    const RPair<R>* rotA = sh.rot[cur][0];
-    const RPair<R>* rotB = sh.rot[cur][1];
-    for (int row = (run && wave != gwave && wave != sib) ? rrank : n; row < n; row += NRT) {
-      R* zr = Z + row;
-      R carryA = zr[(size_t)top * ld];
-      R carryB = 0;
-      auto rotB_at = [&](int k, R zk) {
-        const RPair<R> b = rotB[k];
-        const bool in = static_cast<unsigned>(k - lB) < spanB;
-        const R cb = in ? b.x : R(1), sb = in ? b.y : R(0);
-        zr[(size_t)(k + 1) * ld] = sb * zk + cb * carryB;
-        carryB = cb * zk - sb * carryB;
-      };
-      auto step = [&](int i, R zi) {
-        const RPair<R> a = rotA[i];
-        const bool in = static_cast<unsigned>(i - lA) < spanA;
-        const R ca = in ? a.x : R(1), sa = in ? a.y : R(0);
-        const R e = sa * zi + ca * carryA;  // column i + 1 after sweep A
-        carryA = ca * zi - sa * carryA;
-        if (i == top - 1)
-          carryB = e;  // column top after sweep A starts sweep B's chain
-        else
-          rotB_at(i + 1, e);
-      };
-      R za[CH], zb[CH];
-      auto fetch = [&](R* buf, int i0) {
-#pragma unroll
-        for (int u = 0; u < CH; ++u)
-          if (i0 - u >= bot) buf[u] = zr[(size_t)(i0 - u) * ld];
-      };
-      auto rotate = [&](const R* buf, int i0) {
-#pragma unroll
-        for (int u = 0; u < CH; ++u)
-          if (i0 - u >= bot) step(i0 - u, buf[u]);
-      };
-      fetch(za, top - 1);
-      for (int i0 = top - 1; i0 >= bot; i0 -= 2 * CH) {
-        if (i0 - CH >= bot) fetch(zb, i0 - CH);
-        rotate(za, i0);
-        if (i0 - CH < bot) break;
-        if (i0 - 2 * CH >= bot) fetch(za, i0 - 2 * CH);
-        rotate(zb, i0 - CH);
-      }
-      rotB_at(bot, carryA);  // sweep B's rotation at bot on column bot after sweep A
-      zr[(size_t)bot * ld] = carryB;
-    }
-    __syncthreads();
-  }
-  if (p.prof && wave == gwave && lane == 0) {
-    p.prof[(size_t)sid * 16 + 8] = gen_cycles;
-    p.prof[(size_t)sid * 16 + 9] = n_sweep;
-    p.prof[(size_t)sid * 16 + 10] = n_rot;
-  }
-  if (sh.ql.fail) {
-    for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
-    if (tid == 0) p.info[sid] = -1;  // QL failed to converge
-    __syncthreads();
-    return;
+  // ---- eigenpairs of the real tridiagonal by divide and conquer, eigenvectors into Z ----
+  // (X, free until the back-transform, holds the two n x n real work arrays)
+  {
+    struct WgTeam {
+      int tid;
+      __device__ void sync() const { __syncthreads(); }
+    };
+    WgTeam tm{tid};
+    R* work = reinterpret_cast<R*>(X);
+    dc_tridiag<R, NMAX, NT_>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
+                             work + (size_t)n * n, sh.dc, sh.dct);
   }
   eig_mark(p, sid, 5);
 
