@@ -664,7 +664,8 @@ __device__ __forceinline__ void dft_ct(cplx<R>* v) {
 // Small DFTs with sign +, constant roots.
 template <typename R, int RAD>
 __device__ __forceinline__ void dft_small(cplx<R>* v) {
-  if constexpr (RAD == 2) {
+  if constexpr (RAD == 1) {
+  } else if constexpr (RAD == 2) {
     const cplx<R> a = v[0], b = v[1];
     v[0] = a + b;
     v[1] = a - b;
@@ -963,6 +964,96 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
       if (p.twiddle) v = v * tw2(sub * k);  // sub < n2, k < n1: sub k < ntot
       const int ai = static_cast<int>(sub * p.sub_stride + k * p.estride);
       base[(long long)ai * p.axis_stride + x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Strided axis, m = M1 M2 with both factors radices of dft_small: each transform in two
+// register stages with one LDS exchange (X[k1 + M1 k2] = sum_c2 w_M2^(c2 k2) w_m^(c2 k1)
+// sum_j1 x[c2 + M2 j1] w_M1^(j1 k1)).  Stage 1: thread (x position, c2) loads x[c2 + M2 j1],
+// j1 < M1, straight from global memory, transforms them, applies w_m^(c2 k1) and stores row
+// k1 M2 + c2 of its line in LDS; stage 2: thread (x position, k1) reads the M2 values of row
+// k1, transforms them and stores outputs k1 + M1 k2.  Lanes run over consecutive x
+// positions, so global accesses are contiguous and the odd LDS line stride spreads a
+// wavefront's accesses over the banks.  Same items, windows, pass twiddle and output
+// permutation as fft_pass_kernel.
+template <typename R, int M1, int M2>
+__global__ __launch_bounds__(FFT_NT) void fft_pass_rr_kernel(N3PassParams<R> p, int tw_shift) {
+  extern __shared__ __align__(16) unsigned char fsh[];
+  constexpr int M = M1 * M2;
+  constexpr int LD = M | 1;
+  const int TW = 1 << tw_shift;
+  cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);  // [TW][LD]
+  cplx<R>* twm = a + TW * LD;                     // exp(+2 pi i t / M), t < M
+  cplx<R>* tws2 = twm + M;                        // two-level table of ntot (pass twiddle)
+  const int tid = threadIdx.x;
+  for (int t = tid; t < M; t += FFT_NT) twm[t] = p.tw[t & 63] * p.tw[64 + (t >> 6)];
+  if (p.twiddle) load_twid<R>(p.tw2, static_cast<int>(p.ntot), tws2, tid, FFT_NT);
+  const Twid<R> tw2{tws2, tws2 + 64};
+  __syncthreads();
+  const int n_xt = (p.xw.count + TW - 1) >> tw_shift;
+  const int n_items = n_xt * p.n_sub * p.other.count * p.n_levels;
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int q1 = fast_div(item, p.xt_m, p.xt_s);
+    const int xt = item - q1 * n_xt;
+    const int q2 = fast_div(q1, p.sub_m, p.sub_s);
+    const int sub = q1 - q2 * p.n_sub;
+    const int lv = fast_div(q2, p.oc_m, p.oc_s);
+    const int o = q2 - lv * p.other.count;
+    int oo = p.other.start + o;
+    oo -= oo >= p.no ? p.no : 0;
+    if (p.operm_n2 > 1) {
+      const int q = fast_div(oo, p.op_m, p.op_s);
+      oo = (oo - q * p.operm_n1) * p.operm_n2 + q;
+    }
+    cplx<R>* base = p.grid + (size_t)lv * p.grid_stride + (size_t)oo * p.outer_stride;
+    const long long sub_off = (long long)sub * p.sub_stride;
+    for (int u = tid; u < (M2 << tw_shift); u += FFT_NT) {
+      const int xi = u & (TW - 1), c2 = u >> tw_shift;
+      const int j = (xt << tw_shift) + xi;
+      cplx<R> v[M1];
+      if (j < p.xw.count) {
+        int x = p.xw.start + j;
+        x -= x >= p.nx ? p.nx : 0;
+#pragma unroll
+        for (int j1 = 0; j1 < M1; ++j1) {
+          const int ai = static_cast<int>(sub_off + (long long)(c2 + M2 * j1) * p.estride);
+          v[j1] = (!p.in_check || in_window(ai, p.in, p.nax))
+                      ? base[(long long)ai * p.axis_stride + x]
+                      : mk<R>(0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int j1 = 0; j1 < M1; ++j1) v[j1] = mk<R>(0, 0);
+      }
+      dft_small<R, M1>(v);
+      cplx<R>* row = a + xi * LD + c2;
+      row[0] = v[0];
+#pragma unroll
+      for (int k1 = 1; k1 < M1; ++k1) row[k1 * M2] = c2 == 0 ? v[k1] : v[k1] * twm[c2 * k1];
+    }
+    __syncthreads();
+    for (int u = tid; u < (M1 << tw_shift); u += FFT_NT) {
+      const int xi = u & (TW - 1), k1 = u >> tw_shift;
+      const int j = (xt << tw_shift) + xi;
+      if (j >= p.xw.count) continue;
+      int x = p.xw.start + j;
+      x -= x >= p.nx ? p.nx : 0;
+      cplx<R> v[M2];
+      const cplx<R>* row = a + xi * LD + k1 * M2;
+#pragma unroll
+      for (int c2 = 0; c2 < M2; ++c2) v[c2] = row[c2];
+      dft_small<R, M2>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < M2; ++k2) {
+        const int k = k1 + M1 * k2;
+        if (p.out_check && !in_window(sub * p.fa + k * p.fb, p.out, p.nax)) continue;
+        cplx<R> w = v[k2];
+        if (p.twiddle) w = w * tw2(sub * k);  // sub < n2, k < n1: sub k < ntot
+        const int ai = static_cast<int>(sub_off + (long long)k * p.estride);
+        base[(long long)ai * p.axis_stride + x] = w;
+      }
     }
     __syncthreads();
   }
@@ -1318,6 +1409,72 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
 template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
 template void launch_fft_lines<double>(const N3LineParams<double>&, hipStream_t);
 
+// The two-stage register pass for m = M1 M2: x positions per item a power of two in
+// [16, 2^tw_shift_max] with the line block at most 80 KiB of LDS (two workgroups per CU).
+template <typename R, int M1, int M2>
+void go_pass_rr(const N3PassParams<R>& p, hipStream_t st) {
+  constexpr int M = M1 * M2, LD = M | 1;
+  const int tw_cap = p.tw_shift_max >= 4 && p.tw_shift_max <= 7 ? p.tw_shift_max : 6;
+  int tw_shift = 4;
+  while (tw_shift < tw_cap && (size_t)LD * (2u << tw_shift) * sizeof(cplx<R>) <= 80 * 1024)
+    ++tw_shift;
+  const int TW = 1 << tw_shift;
+  const long long items =
+      (long long)((p.xw.count + TW - 1) / TW) * p.n_sub * p.other.count * p.n_levels;
+  if (items <= 0) return;
+  if (items > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "NUFFT grid too large for one pass");
+  N3PassParams<R> q = p;
+  n3_div_magic((p.xw.count + TW - 1) / TW, q.xt_m, q.xt_s);
+  n3_div_magic(p.n_sub, q.sub_m, q.sub_s);
+  n3_div_magic(p.other.count, q.oc_m, q.oc_s);
+  n3_div_magic(p.operm_n2 > 1 ? p.operm_n1 : 1, q.op_m, q.op_s);
+  const size_t sh = sizeof(cplx<R>) * ((size_t)TW * LD + M +
+                                       (p.twiddle ? twid_len(static_cast<int>(p.ntot)) : 0));
+  if (sh > 96 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT pass exceeds LDS");
+  static bool attr = false;
+  if (!attr) {
+    BIPP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_pass_rr_kernel<R, M1, M2>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+    attr = true;
+  }
+  const unsigned blocks = static_cast<unsigned>(std::min<long long>(items, 32768));
+  fft_pass_rr_kernel<R, M1, M2><<<blocks, FFT_NT, sh, st>>>(q, tw_shift);
+}
+
+// factor pairs compiled for the register pass (M1 >= M2, M1 M2 <= 128)
+#define BIPP_RR_PAIRS(X)                                                                     \
+  X(2, 1) X(2, 2) X(3, 1) X(3, 2) X(3, 3) X(4, 1) X(4, 2) X(4, 3) X(4, 4) X(5, 1) X(5, 2)   \
+  X(5, 3) X(5, 4) X(5, 5) X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(6, 5) X(6, 6) X(8, 1) X(8, 2)   \
+  X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(8, 8) X(9, 1) X(9, 2) X(9, 3) X(9, 4) X(9, 5) X(9, 6)   \
+  X(9, 8) X(9, 9) X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 6) X(16, 8)
+
+template <typename R>
+bool launch_fft_pass_rr(const N3PassParams<R>& p, hipStream_t st) {
+  const int m1 = p.rr_m1, m2 = p.m / std::max(1, p.rr_m1);
+  if (m1 * m2 != p.m) return false;
+#define BIPP_RR_CASE(A, B)              \
+  if (m1 == A && m2 == B) {             \
+    go_pass_rr<R, A, B>(p, st);         \
+    return true;                        \
+  }
+  BIPP_RR_PAIRS(BIPP_RR_CASE)
+#undef BIPP_RR_CASE
+  return false;
+}
+
+bool fft_rr_factors(int m, int& m1, int& m2) {
+#define BIPP_RR_FIND(A, B)   \
+  if (A * B == m) {          \
+    m1 = A;                  \
+    m2 = B;                  \
+    found = true;            \
+  }
+  bool found = false;
+  BIPP_RR_PAIRS(BIPP_RR_FIND)  // the last match: the largest first factor
+#undef BIPP_RR_FIND
+  return found;
+}
+
 template <typename R>
 void launch_fft_pass(const N3PassParams<R>& p_in, hipStream_t st) {
   N3PassParams<R> p = p_in;
@@ -1353,6 +1510,11 @@ void launch_fft_pass(const N3PassParams<R>& p_in, hipStream_t st) {
     }
     kernel<<<blocks, FFT_NT, sh, st>>>(p, tw_shift);
   };
+  if (p.rr_m1 > 0) {
+    if (!launch_fft_pass_rr(p, st)) fail(BIPP_INTERNAL_ERROR, "FFT pass factors not compiled");
+    check_launch("fft_pass_rr_kernel");
+    return;
+  }
   static bool attr5 = false, attr16 = false;
   if (max_radix(p.rad) > 5)
     go(&fft_pass_kernel<R, 16>, attr16);

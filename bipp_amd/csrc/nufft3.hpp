@@ -173,6 +173,7 @@ struct N3PassParams {
   const cplx<R>* tw;             // exp(+2 pi i t / m), t < m
   const cplx<R>* tw2;            // exp(+2 pi i t / ntot), t < ntot (when twiddle)
   int tw_shift_max;               // items hold at most 2^tw_shift_max x positions (4..6)
+  int rr_m1;                      // > 0: two-stage register pass, m = rr_m1 (m / rr_m1)
   // set by the launcher: division multipliers and shifts (see N3Radix) for the item
   // decomposition (x tiles, n_sub, other.count) and for operm_n1
   unsigned xt_m, sub_m, oc_m, op_m;
@@ -236,6 +237,8 @@ template <typename R>
 void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st);
 template <typename R>
 void launch_fft_pass(const N3PassParams<R>& p, hipStream_t st);
+// m = m1 m2 with (m1, m2) a factor pair of the two-stage register pass (m1 >= m2)
+bool fft_rr_factors(int m, int& m1, int& m2);
 template <typename R, typename TI>
 void launch_n3_interp(const N3InterpParams<R>& p, hipStream_t st);
 template <typename TO>

@@ -51,6 +51,14 @@ int fft_pass_radix_max() {
   static const int v = radix_cap("BIPP_FFT_PASS_RADIX_MAX", 4);
   return v;
 }
+// 1: strided passes whose length factors into two dft radices run as two register stages
+int fft_pass_rr_default() {
+  static const int v = [] {
+    const char* e = std::getenv("BIPP_FFT_PASS_RR");
+    return e ? (std::atoi(e) != 0 ? 1 : 0) : 1;
+  }();
+  return v;
+}
 int spread_chunk_default() {
   static const int v = [] {
     const char* e = std::getenv("BIPP_SPREAD_XCD_CHUNK");
@@ -328,6 +336,7 @@ struct N3Tuning {
   int pass_radix = fft_pass_radix_max();
   int pass_tw_shift = 6;
   int spread_chunk = spread_chunk_default();
+  int pass_rr = fft_pass_rr_default();
 };
 
 inline std::vector<N3Tuning> n3_tune_candidates() {
@@ -912,6 +921,10 @@ struct N3Runner {
       fp.nx = nx;
       fp.xw = pl.out_w[0];
       fp.tw_shift_max = cfg.pass_tw_shift;
+      auto rr = [&](int m) {
+        int a = 0, b = 0;
+        return cfg.pass_rr && fft_rr_factors(m, a, b) ? a : 0;
+      };
       if (d == 1) {
         fp.no = nz;
         fp.other = pl.in_w[2];
@@ -943,6 +956,7 @@ struct N3Runner {
         fp.twiddle = 0;
         fp.ntot = n1;
         fp.rad = radices_of(n1, cfg.pass_radix);
+        fp.rr_m1 = rr(n1);
         fp.tw = dtw + tw_off[d][0];
         fp.tw2 = nullptr;
         launch_fft_pass<TG>(fp, st);
@@ -959,6 +973,7 @@ struct N3Runner {
         fp.twiddle = 1;
         fp.ntot = (long long)n1 * n2;
         fp.rad = radices_of(n1, cfg.pass_radix);
+        fp.rr_m1 = rr(n1);
         fp.tw = dtw + tw_off[d][0];
         fp.tw2 = dtw + tw_off[d][2];
         launch_fft_pass<TG>(fp, st);
@@ -972,6 +987,7 @@ struct N3Runner {
         fp.fb = n1;
         fp.twiddle = 0;
         fp.rad = radices_of(n2, cfg.pass_radix);
+        fp.rr_m1 = rr(n2);
         fp.tw = dtw + tw_off[d][1];
         fp.tw2 = nullptr;
         launch_fft_pass<TG>(fp, st);
