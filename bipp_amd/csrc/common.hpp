@@ -32,59 +32,11 @@ inline void hip_check(hipError_t e, const char* what) {
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
-// ---------------------------------------------------------------------------------------
-// Complex arithmetic on plain structs (layout-compatible with std::complex<R>).
-// ---------------------------------------------------------------------------------------
-template <typename R>
-struct alignas(2 * sizeof(R)) cplx {
-  R re, im;
-};
+}  // namespace bippamd
 
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> mk(R re, R im) {
-  cplx<R> c;
-  c.re = re;
-  c.im = im;
-  return c;
-}
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) {
-  return mk(a.re + b.re, a.im + b.im);
-}
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) {
-  return mk(a.re - b.re, a.im - b.im);
-}
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
-  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
-}
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, R s) {
-  return mk(a.re * s, a.im * s);
-}
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> conj(cplx<R> a) {
-  return mk(a.re, -a.im);
-}
-// a * conj(b)
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> mulc(cplx<R> a, cplx<R> b) {
-  return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
-}
-// conj(a) * b
-template <typename R>
-__host__ __device__ __forceinline__ cplx<R> cmul_conj_a(cplx<R> a, cplx<R> b) {
-  return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
-}
-template <typename R>
-__host__ __device__ __forceinline__ R abs2(cplx<R> a) {
-  return a.re * a.re + a.im * a.im;
-}
-template <typename R, typename S>
-__host__ __device__ __forceinline__ cplx<R> cvt(cplx<S> a) {
-  return mk<R>(static_cast<R>(a.re), static_cast<R>(a.im));
-}
+#include "cplx.hpp"
+
+namespace bippamd {
 
 // ---------------------------------------------------------------------------------------
 // Wave64 / workgroup reductions.

@@ -2,7 +2,7 @@
 //
 // Restates, per sample, src/eigensolver.cpp:129-225 (flag -> reduce -> Gram -> ?hegv)
 // with the LAPACK steps of ?hegv(itype=1,'V','L') (src/host/lapack_api.hpp:112-158):
-//   Cholesky G = L L^H (left-looking), W = L^-1, C = W S W^H (two LDS-tiled GEMMs),
+//   Cholesky G = L L^H and W = L^-1 (blocked, chol_inv.hpp), C = W S W^H (two LDS-tiled GEMMs),
 //   Householder tridiagonalisation C = Q T Q^H on the lower triangle, eigenpairs of T by
 //   divide and conquer (tridiag_dc.hpp, eigenvalues ascending, eigenvectors into Z), X = Q Z
 //   (eigenvector columns in registers), eigenvectors W^H X (GEMM).  ?hegv's QL iteration on T
@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 
 #define BIPP_HD __device__
+#include "chol_inv.hpp"
 #include "tridiag_dc.hpp"
 
 #include <limits>
@@ -119,6 +120,7 @@ struct EigShared {
     GemmTiles<R> gt;        // operand tiles of the GEMM passes
     BackTile<R, NMAX> bt;   // reflectors of the back-transform
     DcShared<R, NMAX> dc;   // per-merge arrays of the tridiagonal divide and conquer
+    CiShared<R> ci;         // diagonal blocks and product tiles of the Cholesky inverse
   };
   RPair<R> de[NMAX];      // tridiagonal: (diagonal d_i, subdiagonal e_i)
   int perm[NMAX];
@@ -129,64 +131,6 @@ struct EigShared {
 };
 static_assert(DcCfg<256>::TILE_ELEMS <= 8 * 256 && DcCfg<64>::TILE_ELEMS <= 8 * 64,
               "divide-and-conquer tiles fit in the four vectors");
-
-// --------------------------------------------------------------------------------------
-// W = L^-1 in place on the lower triangle of L (ztrti2, lower, non-unit): columns from the
-// last to the first, W[j+1:n, j] = -W[j+1:n, j+1:n] L[j+1:n, j] / L[j, j], the product
-// taken over the lower triangle only (the strict upper triangle of L is never read).
-// --------------------------------------------------------------------------------------
-template <typename R, int NMAX>
-__device__ void tri_inverse(int n, cplx<R>* __restrict__ L, int ld, EigShared<R, NMAX>& sh) {
-  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
-                UC = EigCfg<NMAX>::UC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int j = n - 1; j >= 0; --j) {
-    const int j0 = j + 1;
-    const R wjj = R(1) / L[j + (size_t)j * ld].re;  // read before tid 0 overwrites it below
-    for (int i = j0 + tid; i < n; i += NT_) sh.v[i] = L[i + (size_t)j * ld];
-    __syncthreads();
-    cplx<R> acc[RM];
-#pragma unroll
-    for (int r = 0; r < RM; ++r) acc[r] = mk<R>(0, 0);
-    for (int kb = j0 + wave; kb < n; kb += NW_ * UC) {
-      cplx<R> a[UC][RM];
-#pragma unroll
-      for (int u = 0; u < UC; ++u) {
-        const int k = kb + u * NW_;
-#pragma unroll
-        for (int r = 0; r < RM; ++r) {
-          const int i = j0 + lane + 64 * r;
-          if (k < n && i < n && i >= k) a[u][r] = L[i + (size_t)k * ld];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < UC; ++u) {
-        const int k = kb + u * NW_;
-        if (k >= n) continue;
-        const cplx<R> vk = sh.v[k];
-#pragma unroll
-        for (int r = 0; r < RM; ++r) {
-          const int i = j0 + lane + 64 * r;
-          if (i < n && i >= k) acc[r] = acc[r] + a[u][r] * vk;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < RM; ++r) {
-      const int i = j0 + lane + 64 * r;
-      if (i < n) sh.xw[wave][i] = acc[r];
-    }
-    __syncthreads();
-    for (int i = j0 + tid; i < n; i += NT_) {
-      cplx<R> sum = sh.xw[0][i];
-#pragma unroll
-      for (int w = 1; w < NW_; ++w) sum = sum + sh.xw[w][i];
-      L[i + (size_t)j * ld] = sum * (-wjj);
-    }
-    if (tid == 0) L[j + (size_t)j * ld] = mk<R>(wjj, 0);
-    __syncthreads();
-  }
-}
 
 // --------------------------------------------------------------------------------------
 // C = op(A) op(B) for n x n matrices, GT x GT output tiles (with LOWER_ONLY the tiles on
@@ -448,6 +392,12 @@ __device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) 
   if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 16 + k] = wall_clock64();
 }
 
+// The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
+struct WgTeam {
+  int tid;
+  __device__ void sync() const { __syncthreads(); }
+};
+
 template <typename R, typename TIn, int NMAX>
 __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
@@ -512,70 +462,21 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   eig_mark(p, sid, 1);
 
   if (!p.standard) {
-    // ---- Cholesky G = L L^H (lower, left-looking) --------------------------------------
-    // column j: y = G[j:n, j] - L[j:n, 0:j] conj(L[j, 0:j])^T (lanes over rows, wavefronts
-    // over the finished columns), L[j, j] = sqrt(y_j), L[j+1:n, j] = y[j+1:n] / L[j, j]:
-    // each finished element is read once per later column and nothing but column j is
-    // written; the strict upper triangle is never read
-    for (int j = 0; j < n; ++j) {
-      for (int k = tid; k < j; k += NT_) sh.v[k] = conj(L[j + (size_t)k * ld]);
+    // ---- W = L^-1 of the Cholesky factor G = L L^H (blocked; X is its scratch) ----------
+    {
+      WgTeam tm{tid};
+      const int bad = chol_inverse<R, NT_>(tm, n, L, ld, X, ldx, sh.ci);
       __syncthreads();
-      cplx<R> acc[RM];
-#pragma unroll
-      for (int r = 0; r < RM; ++r) acc[r] = mk<R>(0, 0);
-      for (int kb = wave; kb < j; kb += NW_ * UC) {
-        cplx<R> a[UC][RM];
-#pragma unroll
-        for (int u = 0; u < UC; ++u) {
-          const int k = kb + u * NW_;
-#pragma unroll
-          for (int r = 0; r < RM; ++r) {
-            const int i = j + lane + 64 * r;
-            if (k < j && i < n) a[u][r] = L[i + (size_t)k * ld];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < UC; ++u) {
-          const int k = kb + u * NW_;
-          if (k >= j) continue;
-          const cplx<R> vk = sh.v[k];
-#pragma unroll
-          for (int r = 0; r < RM; ++r) {
-            const int i = j + lane + 64 * r;
-            if (i < n) acc[r] = acc[r] + a[u][r] * vk;
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RM; ++r) {
-        const int i = j + lane + 64 * r;
-        if (i < n) sh.xw[wave][i] = acc[r];
-      }
-      __syncthreads();
-      for (int i = j + tid; i < n; i += NT_) {
-        cplx<R> y = L[i + (size_t)j * ld];
-#pragma unroll
-        for (int w = 0; w < NW_; ++w) y = y - sh.xw[w][i];
-        sh.w[i] = y;
-      }
-      __syncthreads();
-      const R djj = sh.w[j].re;  // every thread reads the same pivot
-      if (!(djj > R(0) && djj == djj)) {
-        // leading minor j+1 of G is not positive definite: ?hegv reports info = n + j + 1
+      if (bad >= 0) {
+        // leading minor bad + 1 of G is not positive definite: ?hegv reports info = n + bad + 1
         for (int jj = tid; jj < nmax; jj += NT_) dout[jj] = 0;
-        if (tid == 0) p.info[sid] = n + j + 1;
+        if (tid == 0) p.info[sid] = n + bad + 1;
         __syncthreads();
         return;
       }
-      const R sj = sqrt(djj);
-      const R inv = R(1) / sj;
-      for (int i = j + 1 + tid; i < n; i += NT_) L[i + (size_t)j * ld] = sh.w[i] * inv;
-      if (tid == 0) L[j + (size_t)j * ld] = mk<R>(sj, 0);
-      __syncthreads();  // column j complete before the next column stages row j+1
     }
-  eig_mark(p, sid, 2);
-    // ---- C = W S W^H with W = L^-1 -----------------------------------------------------
-    tri_inverse<R, NMAX>(n, L, ld, sh);  // lower triangle of L holds W
+    eig_mark(p, sid, 2);
+    // ---- C = W S W^H -------------------------------------------------------------------
     const cplx<R>* W = L;
     // X = W S (W lower triangular: k <= i)
     gemm_tiles<R, NMAX, false, true, false>(
@@ -653,10 +554,6 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   // ---- eigenpairs of the real tridiagonal by divide and conquer, eigenvectors into Z ----
   // (X, free until the back-transform, holds the two n x n real work arrays)
   {
-    struct WgTeam {
-      int tid;
-      __device__ void sync() const { __syncthreads(); }
-    };
     WgTeam tm{tid};
     R* work = reinterpret_cast<R*>(X);
     dc_tridiag<R, NMAX, NT_>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
