@@ -18,7 +18,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-#define BIPP_HD __device__
+#define BIPP_HD __device__ __forceinline__
 #include "chol_inv.hpp"
 #include "tridiag_dc.hpp"
 
@@ -132,6 +132,12 @@ struct EigShared {
 static_assert(DcCfg<256>::TILE_ELEMS <= 8 * 256 && DcCfg<64>::TILE_ELEMS <= 8 * 64,
               "divide-and-conquer tiles fit in the four vectors");
 
+// The eigensolver's LDS, at namespace scope: the phases kept out of line (own register
+// allocation) name it directly, so that their accesses compile to LDS instructions rather
+// than generic (flat) ones through a reference parameter.
+template <typename R, int NMAX>
+__shared__ EigShared<R, NMAX> eig_sh;
+
 // --------------------------------------------------------------------------------------
 // C = op(A) op(B) for n x n matrices, GT x GT output tiles (with LOWER_ONLY the tiles on
 // and below the diagonal).  fa(i, k) and fb(k, j) return operand elements, zero outside
@@ -143,8 +149,9 @@ static_assert(DcCfg<256>::TILE_ELEMS <= 8 * 256 && DcCfg<64>::TILE_ELEMS <= 8 * 
 // --------------------------------------------------------------------------------------
 template <typename R, int NMAX, bool A_KFAST, bool B_KFAST, bool LOWER_ONLY, typename FA,
           typename FB, typename KR>
-__device__ void gemm_tiles(int n, FA fa, FB fb, KR krange, cplx<R>* __restrict__ C, int ldc,
-                           EigShared<R, NMAX>& sh) {
+__device__ __forceinline__ void gemm_tiles(int n, FA fa, FB fb, KR krange,
+                                           cplx<R>* __restrict__ C, int ldc,
+                                           EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW;
   constexpr int CPW = GT / NW_;
   constexpr int PER = GT * GK / NT_;
@@ -237,8 +244,8 @@ __device__ __forceinline__ void block_sum2(R& a, R& b, EigShared<R, NMAX>& sh) {
 // Householder reflector (zlarfg semantics) annihilating A[c+2:n, c]; writes v (v[c+1]=1)
 // into vec and into A[c+2:n, c], beta into A[c+1, c] and ee[c]; returns tau.
 template <typename R, int NMAX>
-__device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx<R>* vec,
-                             EigShared<R, NMAX>& sh) {
+__device__ __forceinline__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld,
+                                             cplx<R>* vec, EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT;
   const int tid = threadIdx.x;
   cplx<R>* col = A + (size_t)c * ld;
@@ -280,8 +287,9 @@ __device__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld, cplx
 //   conj(A[i, j]) vn_i to x_j (a per-column reduction over the wavefront's lanes).
 // Result x[i0:n) is written to sh.w (v and w are no longer needed afterwards).
 template <typename R, int NMAX, bool APPLY>
-__device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const cplx<R>* v,
-                           const cplx<R>* vn, EigShared<R, NMAX>& sh) {
+__device__ __forceinline__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld,
+                                           const cplx<R>* v, const cplx<R>* vn,
+                                           EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
                 UC = EigCfg<NMAX>::UC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -369,7 +377,8 @@ __device__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld, const
 // w = tau*x - 1/2 tau (tau x)^H v v, for the reflector stored in vec over [c+1, n).
 // On entry sh.w holds x (un-scaled); on exit sh.w holds w.
 template <typename R, int NMAX>
-__device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<R, NMAX>& sh) {
+__device__ __forceinline__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec,
+                                       EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT;
   const int tid = threadIdx.x;
   R pr = 0, pi = 0;
@@ -386,17 +395,34 @@ __device__ void make_w(int n, int c, cplx<R> tau, const cplx<R>* vec, EigShared<
   __syncthreads();
 }
 
-// records the wall clock at phase boundaries when the caller asked for a profile
-template <typename R>
-__device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
-  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 16 + k] = wall_clock64();
-}
-
 // The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
 struct WgTeam {
   int tid;
   __device__ void sync() const { __syncthreads(); }
 };
+
+// W = L^-1 of the Cholesky factor of G (chol_inv.hpp), out of line; -1 or the failing column.
+template <typename R, int NMAX>
+__device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, int ldt) {
+  WgTeam tm{static_cast<int>(threadIdx.x)};
+  return chol_inverse<R, EigCfg<NMAX>::NT>(tm, n, L, ld, T, ldt, eig_sh<R, NMAX>.ci);
+}
+
+// eigenpairs of the tridiagonal in sh.de by divide and conquer (tridiag_dc.hpp), eigenvectors
+// into Z; work: 2 n^2 reals of scratch; out of line
+template <typename R, int NMAX>
+__device__ __noinline__ void dc_phase(int n, R* Z, int ld, R* work) {
+  EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
+  WgTeam tm{static_cast<int>(threadIdx.x)};
+  dc_tridiag<R, NMAX, EigCfg<NMAX>::NT>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
+                                        work + (size_t)n * n, sh.dc, sh.dct);
+}
+
+// records the wall clock at phase boundaries when the caller asked for a profile
+template <typename R>
+__device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
+  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 16 + k] = wall_clock64();
+}
 
 template <typename R, typename TIn, int NMAX>
 __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh) {
@@ -464,8 +490,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   if (!p.standard) {
     // ---- W = L^-1 of the Cholesky factor G = L L^H (blocked; X is its scratch) ----------
     {
-      WgTeam tm{tid};
-      const int bad = chol_inverse<R, NT_>(tm, n, L, ld, X, ldx, sh.ci);
+      const int bad = chol_phase<R, NMAX>(n, L, ld, X, ldx);
       __syncthreads();
       if (bad >= 0) {
         // leading minor bad + 1 of G is not positive definite: ?hegv reports info = n + bad + 1
@@ -553,12 +578,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
 
   // ---- eigenpairs of the real tridiagonal by divide and conquer, eigenvectors into Z ----
   // (X, free until the back-transform, holds the two n x n real work arrays)
-  {
-    WgTeam tm{tid};
-    R* work = reinterpret_cast<R*>(X);
-    dc_tridiag<R, NMAX, NT_>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
-                             work + (size_t)n * n, sh.dc, sh.dct);
-  }
+  dc_phase<R, NMAX>(n, Z, ld, reinterpret_cast<R*>(X));
   eig_mark(p, sid, 5);
 
   // ---- ascending order: rank sort --------------------------------------------------
@@ -704,7 +724,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
 
 template <typename R, typename TIn, int NMAX>
 __global__ __launch_bounds__(EigCfg<NMAX>::NT) void eig_kernel(EigParams<R> p) {
-  __shared__ EigShared<R, NMAX> sh;
+  EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
   __shared__ int job_sh;
   for (;;) {
     if (threadIdx.x == 0) job_sh = static_cast<int>(atomicAdd(p.counter, 1u));
