@@ -382,7 +382,7 @@ def main() -> None:
         achieved = nufft_bytes / (kern_ms * 1e-3) / 1e9
         stage_rl["nufft"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                              "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                             "kernel": "n3_points/scatter, n3_spread_kernel, fft_lines/fft_pass "
+                             "kernel": "n3_points/scatter, n3_spread_kernel, fft_lines/fft_pass_rr "
                                        "(pruned), n3_interp_kernel",
                              "kernel_ms": kern_ms, "fine_grid": list(nf), "spread_grid": nf1,
                              "points": m_pts, "algorithmic_bytes_per_launch": nufft_bytes}
@@ -390,7 +390,7 @@ def main() -> None:
         # (one spread launch per 4 levels; strided axes above 128 points take two passes)
         launches = {"n3_points_kernel": 1, "n3_scatter_kernel": 1,
                     "n3_spread_kernel": (L + 3) // 4, "fft_lines_kernel": 1,
-                    "fft_pass_kernel": sum(2 if x > 128 else 1 for x in nf[1:]),
+                    "fft_pass_rr_kernel": sum(2 if x > 128 else 1 for x in nf[1:]),
                     "n3_interp_kernel": 1}
         per_kernel = {}
         for k, cnt in launches.items():
