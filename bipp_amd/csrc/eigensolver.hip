@@ -401,6 +401,54 @@ struct WgTeam {
   __device__ void sync() const { __syncthreads(); }
 };
 
+// Householder tridiagonalisation (zhetd2 order, lower triangle, n >= 2): d and e into
+// sh.de, the reflectors below the subdiagonal of A with tau in the diagonal slots (0 for the
+// last column).  Each column's pass applies the pending rank-2 update and forms the
+// Hermitian product from the stored lower triangle.  Out of line: its registers are
+// allocated apart from the other phases.
+template <typename R, int NMAX>
+__device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int ld) {
+  EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
+  constexpr int NT_ = EigCfg<NMAX>::NT;
+  const int tid = threadIdx.x;
+  cplx<R>* vcur = sh.v;
+  cplx<R>* vnext = sh.vn;
+  cplx<R> tau = reflector<R, NMAX>(n, 0, A, ld, vcur, sh);
+  if (tid == 0) {
+    sh.de[0].x = A[0].re;
+    A[0] = tau;  // diagonal slot keeps tau_0 for the back-transform
+  }
+  trail_pass<R, NMAX, false>(n, 1, A, ld, vcur, vcur, sh);
+  make_w<R, NMAX>(n, 0, tau, vcur, sh);
+  for (int k = 0; k + 2 < n; ++k) {
+    // pending update (v_k, w_k) applied to column k+1, rows [k+1, n)
+    {
+      const cplx<R> cw = conj(sh.w[k + 1]);
+      const cplx<R> cv = conj(vcur[k + 1]);
+      cplx<R>* col = A + (size_t)(k + 1) * ld;
+      for (int i = k + 1 + tid; i < n; i += NT_) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
+    }
+    __syncthreads();
+    if (tid == 0) sh.de[k + 1].x = A[(k + 1) + (size_t)(k + 1) * ld].re;
+    tau = reflector<R, NMAX>(n, k + 1, A, ld, vnext, sh);
+    if (tid == 0) A[(k + 1) + (size_t)(k + 1) * ld] = tau;
+    trail_pass<R, NMAX, true>(n, k + 2, A, ld, vcur, vnext, sh);
+    make_w<R, NMAX>(n, k + 1, tau, vnext, sh);
+    cplx<R>* t = vcur;
+    vcur = vnext;
+    vnext = t;
+  }
+  if (tid == 0) {
+    cplx<R> a = A[(n - 1) + (size_t)(n - 1) * ld];
+    const cplx<R> vv = vcur[n - 1], ww = sh.w[n - 1];
+    a = a - (vv * conj(ww) + ww * conj(vv));
+    sh.de[n - 1].x = a.re;
+    A[(n - 1) + (size_t)(n - 1) * ld] = mk<R>(0, 0);  // no reflector for the last column
+    sh.de[n - 1].y = 0;
+  }
+  __syncthreads();
+}
+
 // W = L^-1 of the Cholesky factor of G (chol_inv.hpp), out of line; -1 or the failing column.
 template <typename R, int NMAX>
 __device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, int ldt) {
@@ -537,42 +585,7 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     }
     __syncthreads();
   } else {
-    cplx<R>* vcur = sh.v;
-    cplx<R>* vnext = sh.vn;
-    cplx<R> tau = reflector<R, NMAX>(n, 0, A, ld, vcur, sh);
-    if (tid == 0) {
-      sh.de[0].x = A[0].re;
-      A[0] = tau;  // diagonal slot keeps tau_0 for the back-transform
-    }
-    trail_pass<R, NMAX, false>(n, 1, A, ld, vcur, vcur, sh);
-    make_w<R, NMAX>(n, 0, tau, vcur, sh);
-    for (int k = 0; k + 2 < n; ++k) {
-      // pending update (v_k, w_k) applied to column k+1, rows [k+1, n)
-      {
-        const cplx<R> cw = conj(sh.w[k + 1]);
-        const cplx<R> cv = conj(vcur[k + 1]);
-        cplx<R>* col = A + (size_t)(k + 1) * ld;
-        for (int i = k + 1 + tid; i < n; i += NT_) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
-      }
-      __syncthreads();
-      if (tid == 0) sh.de[k + 1].x = A[(k + 1) + (size_t)(k + 1) * ld].re;
-      tau = reflector<R, NMAX>(n, k + 1, A, ld, vnext, sh);
-      if (tid == 0) A[(k + 1) + (size_t)(k + 1) * ld] = tau;
-      trail_pass<R, NMAX, true>(n, k + 2, A, ld, vcur, vnext, sh);
-      make_w<R, NMAX>(n, k + 1, tau, vnext, sh);
-      cplx<R>* t = vcur;
-      vcur = vnext;
-      vnext = t;
-    }
-    if (tid == 0) {
-      cplx<R> a = A[(n - 1) + (size_t)(n - 1) * ld];
-      const cplx<R> vv = vcur[n - 1], ww = sh.w[n - 1];
-      a = a - (vv * conj(ww) + ww * conj(vv));
-      sh.de[n - 1].x = a.re;
-      A[(n - 1) + (size_t)(n - 1) * ld] = mk<R>(0, 0);  // no reflector for the last column
-      sh.de[n - 1].y = 0;
-    }
-    __syncthreads();
+    tridiag_phase<R, NMAX>(n, A, ld);
   }
   eig_mark(p, sid, 4);
 
