@@ -388,7 +388,7 @@ def main() -> None:
                              "points": m_pts, "algorithmic_bytes_per_launch": nufft_bytes}
         # measured HBM bytes of the chain: per-dispatch counters times dispatches per step
         # (one spread launch per 4 levels; strided axes above 128 points take two passes)
-        launches = {"n3_points_kernel": 1, "n3_scatter_kernel": 1,
+        launches = {"n3_points_kernel": 2,
                     "n3_spread_kernel": (L + 3) // 4, "fft_lines_kernel": 1,
                     "fft_pass_rr_kernel": sum(2 if x > 128 else 1 for x in nf[1:]),
                     "n3_interp_kernel": 1}

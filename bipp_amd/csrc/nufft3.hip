@@ -126,7 +126,9 @@ __device__ __forceinline__ bool slot_point(const N3Source& p, int s, long long t
   return true;
 }
 
-// Point records: one thread per listed slot (or per slot of every sample).
+// Point records: one thread per listed slot (or per slot of every sample); counts per bin
+// (fill = 0), or the records and values written into their bins' sorted ranges (fill = 1).
+// Both launches compute a point's bin from the same operations.
 __global__ void n3_points_kernel(N3PointsParams p) {
   const long long sps = p.src.slots_per_sample();
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -135,10 +137,7 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   const int s = static_cast<int>(slot / sps);
   const long long t = static_cast<long long>(slot - (unsigned long long)s * sps);
   SlotPoint sp;
-  if (!slot_point(p.src, s, t, sp)) {
-    p.valid[i] = 0;
-    return;
-  }
+  if (!slot_point(p.src, s, t, sp)) return;
   double cell[3];
   N3Point pt;
   int bin[3];
@@ -155,6 +154,12 @@ __global__ void n3_points_kernel(N3PointsParams p) {
     // index in range (cells outside a block are never written by it)
     bin[d] = min(max(pt.l0[d], 0) / p.bin_size[d], p.nbins[d] - 1);
   }
+  const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
+  if (!p.fill) {
+    atomicAdd(&p.bin_count[binid], 1u);
+    return;
+  }
+  const unsigned dst = p.bin_start[binid] + atomicAdd(&p.bin_fill[binid], 1u);
   double ps, pc;
   if (p.phase_fp32) {
     float fs, fc;
@@ -169,34 +174,30 @@ __global__ void n3_points_kernel(N3PointsParams p) {
     const double f = turns - rint(turns);
     sincospi(2.0 * f, &ps, &pc);
   }
-  const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
   pt.ph[0] = static_cast<float>(pc);
   pt.ph[1] = static_cast<float>(ps);
   pt.src = sp.src;
   pt.mult = sp.mult;
   pt.sample = s;
   pt.flags = sp.conj;
-  p.valid[i] = 1;
-  p.bin_of[i] = binid;
-  p.pts[i] = pt;
+  p.pts[dst] = pt;
   // the point's values of every level, conjugated for an upper-triangle entry
   const float sgn = sp.conj ? -1.f : 1.f;
   if (p.vv_double) {
     const cplx<double>* v = static_cast<const cplx<double>*>(p.vv) + (size_t)s * p.n_levels * p.vv_stride + sp.src;
-    cplx<double>* o = static_cast<cplx<double>*>(p.vals) + (size_t)i * p.n_levels;
+    cplx<double>* o = static_cast<cplx<double>*>(p.vals) + (size_t)dst * p.n_levels;
     for (int lv = 0; lv < p.n_levels; ++lv) {
       const cplx<double> c = v[(size_t)lv * p.vv_stride];
       o[lv] = mk<double>(c.re, sgn * c.im);
     }
   } else {
     const cplx<float>* v = static_cast<const cplx<float>*>(p.vv) + (size_t)s * p.n_levels * p.vv_stride + sp.src;
-    cplx<float>* o = static_cast<cplx<float>*>(p.vals) + (size_t)i * p.n_levels;
+    cplx<float>* o = static_cast<cplx<float>*>(p.vals) + (size_t)dst * p.n_levels;
     for (int lv = 0; lv < p.n_levels; ++lv) {
       const cplx<float> c = v[(size_t)lv * p.vv_stride];
       o[lv] = mk<float>(c.re, sgn * c.im);
     }
   }
-  atomicAdd(&p.bin_count[binid], 1u);
 }
 
 // One workgroup per sample: is its uvw antisymmetric with a zero diagonal?  (Exact float
@@ -316,19 +317,6 @@ __global__ void scan_add_kernel(unsigned* out, const unsigned* block_offsets, in
   if (i < n) out[i] += block_offsets[blockIdx.x];
 }
 
-// counting sort: point records and their values copied to their bin's segment
-__global__ void n3_scatter_kernel(long long n_pts, const unsigned char* valid, const int* bin_of,
-                                  const N3Point* pts, const unsigned long long* vals,
-                                  int val_words, const unsigned* bin_start, unsigned* bin_fill,
-                                  N3Point* sorted, unsigned long long* sorted_vals) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n_pts || !valid[i]) return;
-  const int b = bin_of[i];
-  const unsigned slot = bin_start[b] + atomicAdd(&bin_fill[b], 1u);
-  sorted[slot] = pts[i];
-  for (int k = 0; k < val_words; ++k)
-    sorted_vals[(size_t)slot * val_words + k] = vals[(size_t)i * val_words + k];
-}
 
 // --------------------------------------------------------------------------------------
 // Spreading with output ownership.  One workgroup owns an 8^3 block (cells l + nf1/2 in
@@ -1326,16 +1314,6 @@ void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStr
   }
 }
 
-void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const N3Point* pts, const void* vals, int val_words,
-                       const unsigned* bin_start, unsigned* bin_fill, N3Point* sorted,
-                       void* sorted_vals, hipStream_t st) {
-  if (n_pts <= 0) return;
-  n3_scatter_kernel<<<static_cast<unsigned>((n_pts + 255) / 256), 256, 0, st>>>(
-      n_pts, valid, bin_of, pts, static_cast<const unsigned long long*>(vals), val_words,
-      bin_start, bin_fill, sorted, static_cast<unsigned long long*>(sorted_vals));
-  check_launch("n3_scatter_kernel");
-}
 
 template <typename R, typename TV>
 void launch_n3_spread(const N3SpreadParams<R>& p_all, hipStream_t st) {

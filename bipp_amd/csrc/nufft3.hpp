@@ -85,12 +85,16 @@ struct N3PointsParams {
   const void* vv;                // [nS][n_levels][vv_stride] complex (vv_double: double)
   long long vv_stride;
   int n_levels, vv_double;
-  // outputs, [n_list]
-  unsigned char* valid;
-  int* bin_of;
-  N3Point* pts;
-  void* vals;                    // [n_list][n_levels] complex
-  unsigned* bin_count;
+  // Two launches over the same points: the first (fill = 0) counts the points per bin; the
+  // second (fill = 1), after the scan of the counts into bin_start, writes each point's
+  // record and values straight into its bin's range of the sorted arrays (slot claimed with
+  // bin_fill), so the unsorted records are never stored.
+  int fill;
+  unsigned* bin_count;           // [n_bins] (fill = 0)
+  const unsigned* bin_start;     // [n_bins] (fill = 1)
+  unsigned* bin_fill;            // [n_bins] zeroed (fill = 1)
+  N3Point* pts;                  // [points] sorted by bin (fill = 1)
+  void* vals;                    // [points][n_levels] complex, same order (fill = 1)
 };
 
 // Domain partition of the points over a grid of boxes in u (the uvw-space partition of
@@ -226,11 +230,6 @@ template <typename R>
 void launch_zero(R* p, long long n, hipStream_t st);
 void launch_n3_points(const N3PointsParams& p, int n_samples, hipStream_t st);
 void launch_scan(const unsigned* in, unsigned* out, unsigned* tmp, int n, hipStream_t st);
-// counting sort of the records and their values (val_words 8-byte words per point)
-void launch_n3_scatter(long long n_pts, const unsigned char* valid, const int* bin_of,
-                       const N3Point* pts, const void* vals, int val_words,
-                       const unsigned* bin_start, unsigned* bin_fill, N3Point* sorted,
-                       void* sorted_vals, hipStream_t st);
 template <typename R, typename TV>
 void launch_n3_spread(const N3SpreadParams<R>& p, hipStream_t st);
 template <typename R>

@@ -518,7 +518,7 @@ inline void report_spread_profile(const unsigned long long* dprof, long long nb,
 // level accumulated into `img` (TG per pixel).
 template <typename TG>
 struct N3Runner {
-  DevBuf grid, valid, bin_of, pts, sorted, pvals, sorted_vals;
+  DevBuf grid, sorted, sorted_vals;
   DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid, prof_buf;
   DevBuf group_of, group_count, group_start, group_fill, group_tmp, slot_list, ranges;
   std::vector<cplx<TG>> htw;
@@ -761,18 +761,17 @@ struct N3Runner {
     }
     pp.w = pl.w;
     pp.phase_fp32 = phase_fp32 ? 1 : 0;
-    pp.valid = valid.as<unsigned char>(n_list);
-    pp.bin_of = bin_of.as<int>(n_list);
-    pp.pts = pts.as<N3Point>(n_list);
     pp.vv = vv.p;
     pp.vv_stride = vv_stride;
     pp.n_levels = L;
     pp.vv_double = sizeof(TG) == 8 ? 1 : 0;
-    pp.vals = pvals.as<cplx<TG>>((size_t)n_list * L);
     const long long nb = pl.n_bins();
     if (nb > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many NUFFT bins");
+    // counting sort without an unsorted copy: count per bin, scan, then every point written
+    // into its bin's range
     pp.bin_count = bin_count.as<unsigned>(nb);
     launch_zero<unsigned>(pp.bin_count, nb, st);
+    pp.fill = 0;
     launch_n3_points(pp, nS, st);
     unsigned* bstart = bin_start.as<unsigned>(nb);
     unsigned* btmp = scan_tmp.as<unsigned>(3 * ((size_t)nb / 1024 + 2) + 4096);
@@ -781,8 +780,12 @@ struct N3Runner {
     launch_zero<unsigned>(bfill, nb, st);
     N3Point* srt = sorted.as<N3Point>(n_list);
     cplx<TG>* svals = sorted_vals.as<cplx<TG>>((size_t)n_list * L);
-    launch_n3_scatter(n_list, pp.valid, pp.bin_of, pp.pts, pp.vals,
-                      static_cast<int>(L * sizeof(cplx<TG>) / 8), bstart, bfill, srt, svals, st);
+    pp.fill = 1;
+    pp.bin_start = bstart;
+    pp.bin_fill = bfill;
+    pp.pts = srt;
+    pp.vals = svals;
+    launch_n3_points(pp, nS, st);
     mark0(1);
     // levels in groups whose grids fit the memory bound of one box (at least one level)
     const long long G = pl.grid_cells;
