@@ -171,17 +171,23 @@ struct EigRunner {
       BIPP_HIP(hipStreamSynchronize(st));
       static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "tridiag_eig",
                                      "backtransform", "unwhiten"};
-      double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, sub[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       int cnt = 0;
       for (int k = 0; k < nS; ++k) {
         const unsigned long long* r = &h[(size_t)k * 16];
         if (!r[0] || !r[7]) continue;
         ++cnt;
         for (int q = 0; q < 7; ++q) acc[q] += r[q + 1] >= r[q] && r[q] ? double(r[q + 1] - r[q]) : 0.0;
+        for (int q = 0; q < 8; ++q) sub[q] += double(r[8 + q]);
       }
       std::fprintf(stderr, "[bipp eig profile] %d samples, n_slots %d, us/sample (100 MHz clock):", cnt,
                    n_slots);
       for (int q = 0; q < 7; ++q) std::fprintf(stderr, " %s %.1f", names[q], cnt ? acc[q] / cnt / 100.0 : 0.0);
+      // steps of the divide and conquer, summed over its levels
+      static const char* subn[8] = {"merge order", "deflation", "roots", "z", "norms",
+                                    "rank sort", "M", "products"};
+      std::fprintf(stderr, " | tridiag_eig:");
+      for (int q = 0; q < 8; ++q) std::fprintf(stderr, " %s %.1f", subn[q], cnt ? sub[q] / cnt / 100.0 : 0.0);
       std::fprintf(stderr, "\n");
     }
 

@@ -398,7 +398,23 @@ __device__ __forceinline__ void make_w(int n, int c, cplx<R> tau, const cplx<R>*
 // The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
 struct WgTeam {
   int tid;
+  unsigned long long* prof = nullptr;  // step k's wall-clock time added to prof[k] (thread 0)
+  unsigned long long t = 0;
   __device__ void sync() const { __syncthreads(); }
+  // total over the aligned group of P lanes (a + b == b + a: the same bits in each lane)
+  template <int P, typename R>
+  __device__ R lane_sum(R v) const {
+#pragma unroll
+    for (int m = 1; m < P; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+  }
+  __device__ void mark(int k) {
+    if (prof && tid == 0) {
+      const unsigned long long now = wall_clock64();
+      if (k >= 0) prof[k] += now - t;
+      t = now;
+    }
+  }
 };
 
 // Householder tridiagonalisation (zhetd2 order, lower triangle, n >= 2): d and e into
@@ -459,9 +475,9 @@ __device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, in
 // eigenpairs of the tridiagonal in sh.de by divide and conquer (tridiag_dc.hpp), eigenvectors
 // into Z; work: 2 n^2 reals of scratch; out of line
 template <typename R, int NMAX>
-__device__ __noinline__ void dc_phase(int n, R* Z, int ld, R* work) {
+__device__ __noinline__ void dc_phase(int n, R* Z, int ld, R* work, unsigned long long* prof) {
   EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
-  WgTeam tm{static_cast<int>(threadIdx.x)};
+  WgTeam tm{static_cast<int>(threadIdx.x), prof};
   dc_tridiag<R, NMAX, EigCfg<NMAX>::NT>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
                                         work + (size_t)n * n, sh.dc, sh.dct);
 }
@@ -591,7 +607,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
 
   // ---- eigenpairs of the real tridiagonal by divide and conquer, eigenvectors into Z ----
   // (X, free until the back-transform, holds the two n x n real work arrays)
-  dc_phase<R, NMAX>(n, Z, ld, reinterpret_cast<R*>(X));
+  dc_phase<R, NMAX>(n, Z, ld, reinterpret_cast<R*>(X),
+                    p.prof ? p.prof + (size_t)sid * 16 + 8 : nullptr);
   eig_mark(p, sid, 5);
 
   // ---- ascending order: rank sort --------------------------------------------------

@@ -27,8 +27,10 @@
 //      two half-size products.
 // The matrix is scaled to max |entry| = 1 first and the eigenvalues scaled back at the end.
 //
-// The code runs on a "team" (tid, nt, sync()): the eigensolver's workgroup on the GPU.  Every
-// loop over a team's share is followed by a sync before another thread reads its results.
+// The code runs on a "team" (tid, sync(), mark(k), which may time step k of a level, and
+// lane_sum<P>(v), the total of v over the aligned group of P consecutive threads, the same
+// bits in each): the eigensolver's workgroup on the GPU.  Every loop over a team's share is
+// followed by a sync before another thread reads its results.
 #pragma once
 
 #ifndef BIPP_HD
@@ -70,32 +72,63 @@ struct DcShared {
 };
 
 template <typename R>
-BIPP_HD inline R dc_abs(R x) {
+BIPP_HD R dc_abs(R x) {
   return x < R(0) ? -x : x;
 }
 
 template <typename R>
-BIPP_HD inline R dc_key(R x) {  // NaN ordered as +inf (the sort keys stay a strict order)
+BIPP_HD R dc_key(R x) {  // NaN ordered as +inf (the sort keys stay a strict order)
   return x == x ? x : std::numeric_limits<R>::infinity();
 }
 
 // d_i - lambda_j from the poles and the root's (origin, tau): the same rounding wherever used
 template <typename R>
-BIPP_HD inline R dc_delta(const R* dk, const int* org, const R* tau, int i, int j) {
+BIPP_HD R dc_delta(const R* dk, const int* org, const R* tau, int i, int j) {
   return (dk[i] - dk[org[j]]) - tau[j];
+}
+
+// Roots eta of c eta^2 - a eta + b = 0 (each in the form without cancellation): tn = x + eta
+// for the root with x + eta inside (lo, hi) nearest x; false if neither is.
+template <typename R>
+BIPP_HD bool dc_quad(R c, R a, R b, R x, R lo, R hi, R& tn) {
+  bool have = false;
+  auto take = [&](R eta) {
+    const R t = x + eta;
+    if (t > lo && t < hi && (!have || dc_abs(t - x) < dc_abs(tn - x))) {
+      tn = t;
+      have = true;
+    }
+  };
+  if (c == R(0)) {
+    if (a != R(0)) take(b / a);
+  } else {
+    const R disc = a * a - R(4) * b * c;
+    if (disc >= R(0)) {
+      const R sq = sqrt(disc);
+      const R r1 = a <= R(0) ? (a - sq) / (R(2) * c) : R(2) * b / (a + sq);
+      take(r1);
+      if (r1 != R(0)) take(b / (c * r1));
+    }
+  }
+  return have;
 }
 
 // Root k of 1/rho + sum_i zk_i^2 / (dk_i - lambda) = 0, dk strictly ascending, zk nonzero,
 // rho > 0: lambda_k lies in (dk_k, dk_{k+1}), or in (dk_{K-1}, dk_{K-1} + rho |zk|^2] for
-// the last one.  Returns the nearer pole o (by the sign of the function at the interval's
-// midpoint) and tau = lambda - dk_o.  tau is kept in a bracket narrowed by the sign of the
-// function, which increases with lambda between poles; the step is the root of a two-pole
-// model (poles k and k+1, or K-2 and K-1 for the last root, each pole's partial sum fitted
-// to its value and slope, a constant fitted to the function: R.-C. Li's "middle way"),
-// else a Newton step, else bisection, whichever first stays inside the bracket.
-template <typename R, int MAXIT>
-BIPP_HD void dc_secular(int K, int k, const R* dk, const R* zk, R rho, int& org_out,
-                        R& tau_out) {
+// the last one.  Returns the nearer pole o and tau = lambda - dk_o.  The function increases
+// with lambda between poles; its sign at the interval's midpoint picks the half (and o), and
+// tau is kept in a bracket narrowed by the sign at every iterate.  Start: the root of the
+// two-pole model (poles k and k+1, or K-2 and K-1 for the last root) whose other terms are
+// taken at the midpoint (R.-C. Li's initial guess, the one LAPACK's dlaed4 makes).  Step:
+// the root of the two-pole model with each pole's partial sum fitted to its value and slope
+// and a constant fitted to the function (Li's "middle way"), else a Newton step, else
+// bisection, whichever first stays inside the bracket.
+// The P lanes of a root (part = 0 .. P - 1) each sum the terms i = part, part + P, ...;
+// sum(v) totals a value over the P lanes with the same bits in each, so every lane takes the
+// same steps and returns the same root.
+template <typename R, int MAXIT, int P, class SUM>
+BIPP_HD void dc_secular(int K, int k, int part, const R* dk, const R* zk, R rho, SUM sum,
+                        int& org_out, R& tau_out) {
   const R eps = std::numeric_limits<R>::epsilon();
   if (K == 1) {
     org_out = 0;
@@ -103,12 +136,23 @@ BIPP_HD void dc_secular(int K, int k, const R* dk, const R* zk, R rho, int& org_
     return;
   }
   const R rinv = R(1) / rho;
-  int o, lower;
-  R lo, hi;
+  const int lower = k < K - 1 ? k : K - 2;  // the model's poles: lower, lower + 1
+  R base, mid, lo, hi;                      // the midpoint: dk_base + mid
   if (k < K - 1) {
-    const R mid = R(0.5) * (dk[k + 1] - dk[k]);
-    R f = rinv;
-    for (int i = 0; i < K; ++i) f += zk[i] * zk[i] / ((dk[i] - dk[k]) - mid);
+    base = dk[k];
+    mid = R(0.5) * (dk[k + 1] - dk[k]);
+  } else {
+    R s = 0;
+    for (int i = part; i < K; i += P) s += zk[i] * zk[i];
+    hi = rho * sum(s) * (R(1) + R(4) * eps);
+    base = dk[K - 1];
+    mid = R(0.5) * hi;
+  }
+  R f = 0;
+  for (int i = part; i < K; i += P) f += zk[i] * zk[i] / ((dk[i] - base) - mid);
+  f = rinv + sum(f);
+  int o;
+  if (k < K - 1) {
     if (f >= R(0)) {
       o = k;
       lo = R(0);
@@ -118,66 +162,69 @@ BIPP_HD void dc_secular(int K, int k, const R* dk, const R* zk, R rho, int& org_
       lo = -mid;
       hi = R(0);
     }
-    lower = k;
   } else {
-    R s = 0;
-    for (int i = 0; i < K; ++i) s += zk[i] * zk[i];
     o = K - 1;
-    lo = R(0);
-    hi = rho * s * (R(1) + R(4) * eps);
-    lower = K - 2;
+    if (f >= R(0)) {
+      lo = R(0);
+      hi = mid;
+    } else {
+      lo = mid;
+    }
   }
   const R dorg = dk[o];
-  R tau = R(0.5) * (lo + hi);
+  R tau;
+  {
+    const R z1 = zk[lower] * zk[lower], z2 = zk[lower + 1] * zk[lower + 1];
+    const R p1 = dk[lower] - dorg, p2 = dk[lower + 1] - dorg;
+    // f minus the two poles' terms, as a constant: C + z1/(p1 - t) + z2/(p2 - t) = 0
+    const R cm = f - z1 / ((dk[lower] - base) - mid) - z2 / ((dk[lower + 1] - base) - mid);
+    if (!dc_quad(cm, cm * (p1 + p2) + z1 + z2, cm * p1 * p2 + z1 * p2 + z2 * p1, R(0), lo, hi,
+                 tau))
+      tau = R(0.5) * (lo + hi);
+  }
   for (int it = 0; it < MAXIT; ++it) {
-    R psi = 0, dpsi = 0, phi = 0, dphi = 0, sabs = 0;
-    for (int i = 0; i < K; ++i) {
-      const R del = (dk[i] - dorg) - tau;
-      const R t = zk[i] * zk[i] / del;
-      if (i <= lower) {
-        psi += t;
-        dpsi += t / del;
-      } else {
-        phi += t;
-        dphi += t / del;
-      }
-      sabs += dc_abs(t);
+    // psi: the poles up to lower (terms <= 0), phi: the rest (terms >= 0)
+    R psi = 0, dpsi = 0, phi = 0, dphi = 0;
+    int i = part;
+#pragma unroll 4
+    for (; i <= lower; i += P) {
+      const R rdel = R(1) / ((dk[i] - dorg) - tau);  // one division per term
+      const R t = zk[i] * zk[i] * rdel;
+      psi += t;
+      dpsi += t * rdel;
     }
+#pragma unroll 4
+    for (; i < K; i += P) {
+      const R rdel = R(1) / ((dk[i] - dorg) - tau);
+      const R t = zk[i] * zk[i] * rdel;
+      phi += t;
+      dphi += t * rdel;
+    }
+    psi = sum(psi);
+    dpsi = sum(dpsi);
+    phi = sum(phi);
+    dphi = sum(dphi);
     const R w = rinv + psi + phi;
     const R dw = dpsi + dphi;
     if (w < R(0))
       lo = tau;
     else
       hi = tau;
-    // |w| within the rounding of its evaluation: done
-    if (dc_abs(w) <= R(8) * eps * (rinv + sabs + dc_abs(tau) * dw)) break;
+    // |w| within the rounding of its evaluation (phi - psi: the sum of |terms|): done
+    if (dc_abs(w) <= R(8) * eps * (rinv + phi - psi + dc_abs(tau) * dw)) break;
     const R dl = (dk[lower] - dorg) - tau;
     const R du = (dk[lower + 1] - dorg) - tau;
     // model c eta^2 - a eta + b = 0 (eta: step of tau)
-    const R c = w - dl * dpsi - du * dphi;
-    const R a = (dl + du) * w - dl * du * dw;
-    const R b = dl * du * w;
-    R tn = lo;  // placeholder, replaced below
-    bool have = false;
-    auto take = [&](R eta) {
-      const R t = tau + eta;
-      if (t > lo && t < hi && (!have || dc_abs(t - tau) < dc_abs(tn - tau))) {
+    R tn;
+    bool have = dc_quad(w - dl * dpsi - du * dphi, (dl + du) * w - dl * du * dw, dl * du * w,
+                        tau, lo, hi, tn);
+    if (!have && dw > R(0)) {
+      const R t = tau - w / dw;
+      if (t > lo && t < hi) {
         tn = t;
         have = true;
       }
-    };
-    if (c == R(0)) {
-      if (a != R(0)) take(b / a);
-    } else {
-      const R disc = a * a - R(4) * b * c;
-      if (disc >= R(0)) {
-        const R sq = sqrt(disc);
-        const R r1 = a <= R(0) ? (a - sq) / (R(2) * c) : R(2) * b / (a + sq);
-        take(r1);
-        if (r1 != R(0)) take(b / (c * r1));
-      }
     }
-    if (!have && dw > R(0)) take(-w / dw);
     if (!have) tn = R(0.5) * (lo + hi);
     if (!(tn > lo && tn < hi) || tn == tau) break;  // bracket at its resolution
     tau = tn;
@@ -188,7 +235,7 @@ BIPP_HD void dc_secular(int K, int k, const R* dk, const R* zk, R rho, int& org_
 
 // Number of keys in sorted d[b .. e) below x (strict) or not above x (!strict).
 template <typename R>
-BIPP_HD inline int dc_count(const R* d, int st, int b, int e, R x, bool strict) {
+BIPP_HD int dc_count(const R* d, int st, int b, int e, R x, bool strict) {
   int lo = b, hi = e;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -278,6 +325,8 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
                         DcShared<R, NMAX>& sh, R* tile) {
   constexpr int TR = DcCfg<NMAX>::TR;
   constexpr int MAXIT = DcCfg<NMAX>::MAXIT;
+  // lanes per secular root: a level has at most n <= NMAX roots
+  constexpr int RL = NT >= 4 * NMAX ? 4 : NT >= 2 * NMAX ? 2 : 1;
   const int tid = tm.tid;
   const R eps = std::numeric_limits<R>::epsilon();
   const R rsqrt2 = R(0.70710678118654752440);
@@ -323,6 +372,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
     tm.sync();
   }
 
+  tm.mark(-1);
   for (int l = 1; l <= L; ++l) {
     const int h = 1 << (l - 1), S = 1 << l;
     const int nm = (n + S - 1) >> l;
@@ -350,6 +400,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       sh.slot_of[i] = slot;
     }
     tm.sync();
+    tm.mark(0);
 
     // ---- 2. deflation, one thread per merge --------------------------------------------
     for (int m = tid; m < nm; m += NT) {
@@ -409,9 +460,10 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       sh.mR[m] = nr;
     }
     tm.sync();
+    tm.mark(1);
 
-    // ---- 3. roots, one thread each -------------------------------------------------------
-    for (int j = tid; j < n; j += NT) {
+    // ---- 3. roots, RL lanes each -------------------------------------------------------
+    for (int j = tid / RL; j < n; j += NT / RL) {
       const int m = j >> l, r0 = m << l;
       if (r0 + h >= n) continue;
       const int K = sh.mK[m], k = j - r0;
@@ -419,11 +471,15 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       const R rho = R(2) * dc_abs(e[(long long)(r0 + h - 1) * st]);
       int o;
       R t;
-      dc_secular<R, MAXIT>(K, k, sh.dk + r0, sh.zk + r0, rho, o, t);
-      sh.org[j] = o;
-      sh.tau[j] = t;
+      dc_secular<R, MAXIT, RL>(K, k, tid % RL, sh.dk + r0, sh.zk + r0, rho,
+                               [&](R v) { return tm.template lane_sum<RL>(v); }, o, t);
+      if (tid % RL == 0) {
+        sh.org[j] = o;
+        sh.tau[j] = t;
+      }
     }
     tm.sync();
+    tm.mark(2);
 
     // ---- 4. recomputed z ------------------------------------------------------------------
     for (int i = tid; i < n; i += NT) {
@@ -440,6 +496,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       sh.zh[i] = copysign(sqrt(dc_abs(prod)), sh.zk[i]);
     }
     tm.sync();
+    tm.mark(3);
 
     // ---- 5a. vector norms; item values per slot ----------------------------------------
     for (int j = tid; j < n; j += NT) {
@@ -465,6 +522,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       sh.zs[p] = k < 0 ? sh.ds[p] : sh.dk[r0 + sh.org[r0 + k]] + sh.tau[r0 + k];
     }
     tm.sync();
+    tm.mark(4);
 
     // ---- 5b. output order (rank sort of the items, ties by slot) -----------------------
     for (int p = tid; p < n; p += NT) {
@@ -481,6 +539,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       d[(long long)(r0 + rank) * st] = sh.zs[p];
     }
     tm.sync();
+    tm.mark(5);
 
     // ---- 5c. M in slot space, row-major [slot][column] at M + r0 S ----------------------
     for (long long g = tid; g < (long long)n * S; g += NT) {
@@ -523,6 +582,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       }
     }
     tm.sync();
+    tm.mark(6);
 
     // ---- 6. Q_new = diag(Q1, Q2) M ---------------------------------------------------------
     // blocks without a sibling are copied
@@ -562,6 +622,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       }
     }
     tm.sync();
+    tm.mark(7);
   }
 
   for (int i = tid; i < n; i += NT) d[(long long)i * st] *= scale;

@@ -9,6 +9,7 @@
 #include <barrier>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -21,7 +22,26 @@ namespace {
 struct CpuTeam {
   int tid;
   std::barrier<>* bar;
+  int gsize;                                             // lanes per lane_sum group
+  std::vector<std::unique_ptr<std::barrier<>>>* groups;  // one barrier per group
+  double* slot;                                          // one value per thread
   void sync() { bar->arrive_and_wait(); }
+  void mark(int) {}
+  // the GPU's butterfly over the group: v += v of lane tid ^ m for m = 1, 2, ...
+  template <int P, typename R>
+  R lane_sum(R v) {
+    if (P == 1) return v;
+    if (P != gsize) std::abort();
+    std::barrier<>& gb = *(*groups)[tid / P];
+    for (int m = 1; m < P; m <<= 1) {
+      slot[tid] = static_cast<double>(v);
+      gb.arrive_and_wait();
+      const R o = static_cast<R>(slot[tid ^ m]);
+      gb.arrive_and_wait();
+      v += o;
+    }
+    return v;
+  }
 };
 
 template <typename R, int NMAX>
@@ -34,10 +54,14 @@ void solve(int n, std::vector<double>& d64, std::vector<double>& e64, std::vecto
   auto sh = std::make_unique<bippamd::DcShared<R, NMAX>>();
   std::vector<R> tile(bippamd::DcCfg<NMAX>::TILE_ELEMS);
   std::barrier<> bar(NT);
+  constexpr int G = NT >= 4 * NMAX ? 4 : NT >= 2 * NMAX ? 2 : 1;  // lanes per secular root
+  std::vector<std::unique_ptr<std::barrier<>>> groups;
+  for (int g = 0; g < NT / G; ++g) groups.push_back(std::make_unique<std::barrier<>>(G));
+  std::vector<double> slot(NT);
   std::vector<std::thread> th;
   for (int t = 0; t < NT; ++t)
     th.emplace_back([&, t] {
-      CpuTeam tm{t, &bar};
+      CpuTeam tm{t, &bar, G, &groups, slot.data()};
       bippamd::dc_tridiag<R, NMAX, NT>(tm, n, d.data(), e.data(), 1, Z.data(), n, W.data(),
                                        M.data(), *sh, tile.data());
     });
