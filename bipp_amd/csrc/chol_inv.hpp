@@ -18,10 +18,25 @@
 #ifndef BIPP_HD
 #define BIPP_HD __host__ __device__
 #endif
+#ifndef BIPP_GLOBAL
+#define BIPP_GLOBAL  // pointee qualifier of the matrices (the device build: global memory)
+#endif
 
 #include "cplx.hpp"
 
 namespace bippamd {
+
+// read and write a matrix element member by member (a struct in another address space has
+// no copy constructor or assignment of its own)
+template <typename R>
+BIPP_HD cplx<R> ci_get(const BIPP_GLOBAL cplx<R>& x) {
+  return mk<R>(x.re, x.im);
+}
+template <typename R>
+BIPP_HD void ci_put(BIPP_GLOBAL cplx<R>& x, cplx<R> v) {
+  x.re = v.re;
+  x.im = v.im;
+}
 
 constexpr int CI_CB = 32;  // diagonal block edge
 constexpr int CI_T = 64;   // product tile edge
@@ -114,17 +129,17 @@ BIPP_HD void ci_gemm(Team& tm, int m, int nn, FA fa, FB fb, KR krange, ST store,
 // not positive (the leading minor of order j + 1 is not positive definite; L is then left
 // partly transformed).
 template <typename R, int NT, class Team>
-BIPP_HD int chol_inverse(Team& tm, int n, cplx<R>* L, int ld, cplx<R>* T, int ldt,
+BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_GLOBAL cplx<R>* T, int ldt,
                          CiShared<R>& sh) {
   const int tid = tm.tid;
-  auto at = [=](int i, int j) -> cplx<R>& { return L[i + (long long)j * ld]; };
+  auto at = [=](int i, int j) -> BIPP_GLOBAL cplx<R>& { return L[i + (long long)j * ld]; };
 
   // ---- 1. blocked right-looking factorisation, diagonal blocks inverted -----------------
   for (int j0 = 0; j0 < n; j0 += CI_CB) {
     const int jb = n - j0 < CI_CB ? n - j0 : CI_CB;
     for (int e = tid; e < CI_CB * CI_CB; e += NT) {
       const int i = e % CI_CB, j = e / CI_CB;
-      sh.blk.d[i][j] = (i < jb && i >= j) ? at(j0 + i, j0 + j) : mk<R>(0, 0);
+      sh.blk.d[i][j] = (i < jb && i >= j) ? ci_get(at(j0 + i, j0 + j)) : mk<R>(0, 0);
     }
     tm.sync();
     // step k: pivot d_kk (final after step k - 1), trailing columns updated with the
@@ -171,7 +186,7 @@ BIPP_HD int chol_inverse(Team& tm, int n, cplx<R>* L, int ld, cplx<R>* T, int ld
     tm.sync();
     for (int e = tid; e < CI_CB * CI_CB; e += NT) {
       const int i = e % CI_CB, j = e / CI_CB;
-      if (i < jb && i >= j) at(j0 + i, j0 + j) = sh.blk.w[i][j];
+      if (i < jb && i >= j) ci_put(at(j0 + i, j0 + j), sh.blk.w[i][j]);
     }
     tm.sync();
     const int mp = n - j0 - jb;
@@ -179,27 +194,27 @@ BIPP_HD int chol_inverse(Team& tm, int n, cplx<R>* L, int ld, cplx<R>* T, int ld
       const int r0 = j0 + jb;
       // L_{>J,J} = G_{>J,J} W_JJ^H in place (a tile reads its rows' k < jb before it stores)
       ci_gemm<R, NT, false, false>(
-          tm, mp, jb, [=](int i, int k) { return L[(r0 + i) + (long long)(j0 + k) * ld]; },
+          tm, mp, jb, [=](int i, int k) { return ci_get(L[(r0 + i) + (long long)(j0 + k) * ld]); },
           [=](int k, int j) {
-            return k <= j ? conj(L[(j0 + j) + (long long)(j0 + k) * ld]) : mk<R>(0, 0);
+            return k <= j ? conj(ci_get(L[(j0 + j) + (long long)(j0 + k) * ld])) : mk<R>(0, 0);
           },
           [=](int, int, int& klo, int& khi) {
             klo = 0;
             khi = jb;
           },
-          [=](int i, int j, cplx<R> v) { L[(r0 + i) + (long long)(j0 + j) * ld] = v; }, sh);
+          [=](int i, int j, cplx<R> v) { ci_put(L[(r0 + i) + (long long)(j0 + j) * ld], v); }, sh);
       // G_{>J,>J} -= L_{>J,J} L_{>J,J}^H on and below the diagonal
       ci_gemm<R, NT, true, false>(
-          tm, mp, mp, [=](int i, int k) { return L[(r0 + i) + (long long)(j0 + k) * ld]; },
-          [=](int k, int j) { return conj(L[(r0 + j) + (long long)(j0 + k) * ld]); },
+          tm, mp, mp, [=](int i, int k) { return ci_get(L[(r0 + i) + (long long)(j0 + k) * ld]); },
+          [=](int k, int j) { return conj(ci_get(L[(r0 + j) + (long long)(j0 + k) * ld])); },
           [=](int, int, int& klo, int& khi) {
             klo = 0;
             khi = jb;
           },
           [=](int i, int j, cplx<R> v) {
             if (i >= j) {
-              cplx<R>& c = L[(r0 + i) + (long long)(r0 + j) * ld];
-              c = c - v;
+              BIPP_GLOBAL cplx<R>& c = L[(r0 + i) + (long long)(r0 + j) * ld];
+              ci_put(c, ci_get(c) - v);
             }
           },
           sh);
@@ -213,28 +228,28 @@ BIPP_HD int chol_inverse(Team& tm, int n, cplx<R>* L, int ld, cplx<R>* T, int ld
       const int b1 = b0 + s;
       // T = L21 W11 (W11 lower: k >= j)
       ci_gemm<R, NT, false, true>(
-          tm, s2, s, [=](int i, int k) { return L[(b1 + i) + (long long)(b0 + k) * ld]; },
+          tm, s2, s, [=](int i, int k) { return ci_get(L[(b1 + i) + (long long)(b0 + k) * ld]); },
           [=](int k, int j) {
-            return k >= j ? L[(b0 + k) + (long long)(b0 + j) * ld] : mk<R>(0, 0);
+            return k >= j ? ci_get(L[(b0 + k) + (long long)(b0 + j) * ld]) : mk<R>(0, 0);
           },
           [=](int, int jt, int& klo, int& khi) {
             klo = jt;
             khi = s;
           },
-          [=](int i, int j, cplx<R> v) { T[i + (long long)j * ldt] = v; }, sh);
+          [=](int i, int j, cplx<R> v) { ci_put(T[i + (long long)j * ldt], v); }, sh);
       // W21 = -W22 T (W22 lower: k <= i), over L21
       ci_gemm<R, NT, false, true>(
           tm, s2, s,
           [=](int i, int k) {
-            return k <= i ? L[(b1 + i) + (long long)(b1 + k) * ld] : mk<R>(0, 0);
+            return k <= i ? ci_get(L[(b1 + i) + (long long)(b1 + k) * ld]) : mk<R>(0, 0);
           },
-          [=](int k, int j) { return T[k + (long long)j * ldt]; },
+          [=](int k, int j) { return ci_get(T[k + (long long)j * ldt]); },
           [=](int it, int, int& klo, int& khi) {
             klo = 0;
             khi = it + CI_T < s2 ? it + CI_T : s2;
           },
           [=](int i, int j, cplx<R> v) {
-            L[(b1 + i) + (long long)(b0 + j) * ld] = mk<R>(-v.re, -v.im);
+            ci_put(L[(b1 + i) + (long long)(b0 + j) * ld], mk<R>(-v.re, -v.im));
           },
           sh);
     }

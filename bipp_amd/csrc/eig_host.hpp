@@ -160,25 +160,26 @@ struct EigRunner {
     // BIPP_EIG_PROFILE=1: per-phase wall-clock averages of this launch on stderr
     const bool prof = std::getenv("BIPP_EIG_PROFILE") != nullptr;
     if (prof) {
-      ep.prof = prof_buf.as<unsigned long long>((size_t)nS * 16);
-      BIPP_HIP(hipMemsetAsync(ep.prof, 0, sizeof(unsigned long long) * nS * 16, st));
+      ep.prof = prof_buf.as<unsigned long long>((size_t)nS * EIG_PROF_SLOTS);
+      BIPP_HIP(hipMemsetAsync(ep.prof, 0, sizeof(unsigned long long) * nS * EIG_PROF_SLOTS, st));
     }
     launch_eig<R>(ep, n_slots, s_double, st);
     if (prof) {
-      std::vector<unsigned long long> h((size_t)nS * 16);
+      std::vector<unsigned long long> h((size_t)nS * EIG_PROF_SLOTS);
       BIPP_HIP(hipMemcpyAsync(h.data(), ep.prof, sizeof(unsigned long long) * h.size(),
                               hipMemcpyDeviceToHost, st));
       BIPP_HIP(hipStreamSynchronize(st));
       static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "tridiag_eig",
                                      "backtransform", "unwhiten"};
-      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, sub[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, sub[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tsub[3] = {0, 0, 0};
       int cnt = 0;
       for (int k = 0; k < nS; ++k) {
-        const unsigned long long* r = &h[(size_t)k * 16];
+        const unsigned long long* r = &h[(size_t)k * EIG_PROF_SLOTS];
         if (!r[0] || !r[7]) continue;
         ++cnt;
         for (int q = 0; q < 7; ++q) acc[q] += r[q + 1] >= r[q] && r[q] ? double(r[q + 1] - r[q]) : 0.0;
         for (int q = 0; q < 8; ++q) sub[q] += double(r[8 + q]);
+        for (int q = 0; q < 3; ++q) tsub[q] += double(r[16 + q]);
       }
       std::fprintf(stderr, "[bipp eig profile] %d samples, n_slots %d, us/sample (100 MHz clock):", cnt,
                    n_slots);
@@ -188,6 +189,10 @@ struct EigRunner {
                                     "rank sort", "M", "products"};
       std::fprintf(stderr, " | tridiag_eig:");
       for (int q = 0; q < 8; ++q) std::fprintf(stderr, " %s %.1f", subn[q], cnt ? sub[q] / cnt / 100.0 : 0.0);
+      // steps of the tridiagonalisation, summed over its columns
+      static const char* tsn[3] = {"column and reflector", "pass", "w"};
+      std::fprintf(stderr, " | tridiag:");
+      for (int q = 0; q < 3; ++q) std::fprintf(stderr, " %s %.1f", tsn[q], cnt ? tsub[q] / cnt / 100.0 : 0.0);
       std::fprintf(stderr, "\n");
     }
 

@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 
 #define BIPP_HD __device__ __forceinline__
+#define BIPP_GLOBAL __attribute__((address_space(1)))
 #include "chol_inv.hpp"
 #include "tridiag_dc.hpp"
 
@@ -241,17 +242,67 @@ __device__ __forceinline__ void block_sum2(R& a, R& b, EigShared<R, NMAX>& sh) {
   }
 }
 
+// Column-major matrix of cplx<R> (leading dimension ld) accessed by raw buffer
+// instructions.  They count only against the vector-memory counter (through the generic
+// pointer of an out-of-line function they would be flat accesses, which every LDS wait also
+// waits for), and an offset past the buffer's end (OOB) reads zero and drops the store, so a
+// masked lane needs no branch around its load or store.
+template <typename R>
+struct BufMat {
+  static constexpr unsigned OOB = 0x80000000u;
+  __amdgpu_buffer_rsrc_t rs;
+  int ld;
+  __device__ BufMat(cplx<R>* A, int ld_, int n)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(
+            A, 0, (int)(sizeof(cplx<R>) * ((size_t)ld_ * (n - 1) + n)), 0x00020000)),
+        ld(ld_) {}
+  __device__ unsigned off(int i, int j) const {
+    return (unsigned)(sizeof(cplx<R>) * (i + (size_t)j * ld));
+  }
+  __device__ cplx<R> load(unsigned o) const;
+  __device__ void store(unsigned o, cplx<R> x) const;
+};
+template <>
+__device__ __forceinline__ cplx<double> BufMat<double>::load(unsigned o) const {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0);
+  return mk<double>(__builtin_bit_cast(double, ((unsigned long long)v.y << 32) | v.x),
+                    __builtin_bit_cast(double, ((unsigned long long)v.w << 32) | v.z));
+}
+template <>
+__device__ __forceinline__ void BufMat<double>::store(unsigned o, cplx<double> x) const {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const unsigned long long a = __builtin_bit_cast(unsigned long long, x.re);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x.im);
+  const u4 v = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, rs, o, 0, 0);
+}
+// two 32-bit loads: this compiler lowers the 64-bit buffer load builtin to a single dword
+template <>
+__device__ __forceinline__ cplx<float> BufMat<float>::load(unsigned o) const {
+  return mk<float>(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0)),
+                   __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o + 4, 0, 0)));
+}
+template <>
+__device__ __forceinline__ void BufMat<float>::store(unsigned o, cplx<float> x) const {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const u2 v = {__builtin_bit_cast(unsigned, x.re), __builtin_bit_cast(unsigned, x.im)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, rs, o, 0, 0);
+}
+
 // Householder reflector (zlarfg semantics) annihilating A[c+2:n, c]; writes v (v[c+1]=1)
-// into vec and into A[c+2:n, c], beta into A[c+1, c] and ee[c]; returns tau.
+// into vec and into A[c+2:n, c], beta into A[c+1, c] and ee[c]; returns tau.  Thread t holds
+// row c + 2 + t.
 template <typename R, int NMAX>
-__device__ __forceinline__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__ A, int ld,
-                                             cplx<R>* vec, EigShared<R, NMAX>& sh) {
-  constexpr int NT_ = EigCfg<NMAX>::NT;
+__device__ __forceinline__ cplx<R> reflector(int n, int c, const BufMat<R>& M, cplx<R>* vec,
+                                             EigShared<R, NMAX>& sh) {
+  static_assert(EigCfg<NMAX>::NT >= NMAX, "one row per thread");
   const int tid = threadIdx.x;
-  cplx<R>* col = A + (size_t)c * ld;
-  const cplx<R> alpha = col[c + 1];
-  R part = 0, unused = 0;
-  for (int i = c + 2 + tid; i < n; i += NT_) part += abs2(col[i]);
+  const cplx<R> alpha = M.load(M.off(c + 1, c));
+  const int i = c + 2 + tid;
+  const unsigned oi = i < n ? M.off(i, c) : BufMat<R>::OOB;
+  const cplx<R> xi = M.load(oi);
+  R part = abs2(xi), unused = 0;
   block_sum2(part, unused, sh);  // sh.red is next written after this function's barrier
   const R xnorm2 = part;
   cplx<R> tau, scal;
@@ -266,14 +317,12 @@ __device__ __forceinline__ cplx<R> reflector(int n, int c, cplx<R>* __restrict__
     tau = mk<R>((beta - alpha.re) / beta, -alpha.im / beta);
     scal = crecip(mk<R>(alpha.re - beta, alpha.im));
   }
-  for (int i = c + 2 + tid; i < n; i += NT_) {
-    cplx<R> vi = col[i] * scal;
-    col[i] = vi;
-    vec[i] = vi;
-  }
+  const cplx<R> vi = xi * scal;
+  M.store(oi, vi);
+  if (i < n) vec[i] = vi;
   if (tid == 0) {
     vec[c + 1] = mk<R>(1, 0);
-    col[c + 1] = mk<R>(beta, 0);
+    M.store(M.off(c + 1, c), mk<R>(beta, 0));
     sh.de[c].y = beta;
   }
   __syncthreads();
@@ -395,6 +444,26 @@ __device__ __forceinline__ void make_w(int n, int c, cplx<R> tau, const cplx<R>*
   __syncthreads();
 }
 
+// Arguments of an out-of-line function arrive in vector registers.  The phases' sizes and
+// pointers are the same in every lane: read back from the first lane they are known to be
+// uniform, so the loops over them branch on scalars and buffer resources built from them stay
+// in scalar registers.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T>
+__device__ __forceinline__ T* uniform(T* p) {
+  const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(a)));
+  const unsigned hi =
+      static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(a >> 32)));
+  return reinterpret_cast<T*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+// A pointer into global memory as such (a generic pointer gives flat instructions, which
+// count against the LDS counter as well, so every LDS wait would also wait for them).
+template <typename T>
+__device__ __forceinline__ BIPP_GLOBAL T* global_ptr(T* p) {
+  return (BIPP_GLOBAL T*)uniform(p);
+}
+
 // The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
 struct WgTeam {
   int tid;
@@ -411,7 +480,7 @@ struct WgTeam {
   __device__ void mark(int k) {
     if (prof && tid == 0) {
       const unsigned long long now = wall_clock64();
-      if (k >= 0) prof[k] += now - t;
+      if (k >= 0) ((BIPP_GLOBAL unsigned long long*)prof)[k] += now - t;
       t = now;
     }
   }
@@ -423,43 +492,56 @@ struct WgTeam {
 // Hermitian product from the stored lower triangle.  Out of line: its registers are
 // allocated apart from the other phases.
 template <typename R, int NMAX>
-__device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int ld) {
+__device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int ld,
+                                           unsigned long long* prof) {
+  n = uniform(n);
+  ld = uniform(ld);
+  A = uniform(A);
+  prof = uniform(prof);
   EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
-  constexpr int NT_ = EigCfg<NMAX>::NT;
   const int tid = threadIdx.x;
   cplx<R>* vcur = sh.v;
   cplx<R>* vnext = sh.vn;
-  cplx<R> tau = reflector<R, NMAX>(n, 0, A, ld, vcur, sh);
+  const BufMat<R> M(A, ld, n);
+  WgTeam tm{tid, prof};  // step timing: 0 column update and reflector, 1 pass, 2 w
+  tm.mark(-1);
+  const R d0 = M.load(M.off(0, 0)).re;
+  cplx<R> tau = reflector<R, NMAX>(n, 0, M, vcur, sh);
   if (tid == 0) {
-    sh.de[0].x = A[0].re;
-    A[0] = tau;  // diagonal slot keeps tau_0 for the back-transform
+    sh.de[0].x = d0;
+    M.store(M.off(0, 0), tau);  // diagonal slot keeps tau_0 for the back-transform
   }
   trail_pass<R, NMAX, false>(n, 1, A, ld, vcur, vcur, sh);
   make_w<R, NMAX>(n, 0, tau, vcur, sh);
   for (int k = 0; k + 2 < n; ++k) {
-    // pending update (v_k, w_k) applied to column k+1, rows [k+1, n)
+    // pending update (v_k, w_k) applied to column k+1, rows [k+1, n): row k + 1 + tid
     {
-      const cplx<R> cw = conj(sh.w[k + 1]);
-      const cplx<R> cv = conj(vcur[k + 1]);
-      cplx<R>* col = A + (size_t)(k + 1) * ld;
-      for (int i = k + 1 + tid; i < n; i += NT_) col[i] = col[i] - (vcur[i] * cw + sh.w[i] * cv);
+      const int i = k + 1 + tid;
+      if (i < n) {
+        const unsigned o = M.off(i, k + 1);
+        const cplx<R> x = M.load(o) - (vcur[i] * conj(sh.w[k + 1]) + sh.w[i] * conj(vcur[k + 1]));
+        M.store(o, x);
+        if (tid == 0) sh.de[k + 1].x = x.re;
+      }
     }
     __syncthreads();
-    if (tid == 0) sh.de[k + 1].x = A[(k + 1) + (size_t)(k + 1) * ld].re;
-    tau = reflector<R, NMAX>(n, k + 1, A, ld, vnext, sh);
-    if (tid == 0) A[(k + 1) + (size_t)(k + 1) * ld] = tau;
+    tau = reflector<R, NMAX>(n, k + 1, M, vnext, sh);
+    if (tid == 0) M.store(M.off(k + 1, k + 1), tau);
+    tm.mark(0);
     trail_pass<R, NMAX, true>(n, k + 2, A, ld, vcur, vnext, sh);
+    tm.mark(1);
     make_w<R, NMAX>(n, k + 1, tau, vnext, sh);
+    tm.mark(2);
     cplx<R>* t = vcur;
     vcur = vnext;
     vnext = t;
   }
   if (tid == 0) {
-    cplx<R> a = A[(n - 1) + (size_t)(n - 1) * ld];
+    cplx<R> a = M.load(M.off(n - 1, n - 1));
     const cplx<R> vv = vcur[n - 1], ww = sh.w[n - 1];
     a = a - (vv * conj(ww) + ww * conj(vv));
     sh.de[n - 1].x = a.re;
-    A[(n - 1) + (size_t)(n - 1) * ld] = mk<R>(0, 0);  // no reflector for the last column
+    M.store(M.off(n - 1, n - 1), mk<R>(0, 0));  // no reflector for the last column
     sh.de[n - 1].y = 0;
   }
   __syncthreads();
@@ -469,7 +551,8 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
 template <typename R, int NMAX>
 __device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, int ldt) {
   WgTeam tm{static_cast<int>(threadIdx.x)};
-  return chol_inverse<R, EigCfg<NMAX>::NT>(tm, n, L, ld, T, ldt, eig_sh<R, NMAX>.ci);
+  return chol_inverse<R, EigCfg<NMAX>::NT>(tm, uniform(n), global_ptr(L), uniform(ld),
+                                           global_ptr(T), uniform(ldt), eig_sh<R, NMAX>.ci);
 }
 
 // eigenpairs of the tridiagonal in sh.de by divide and conquer (tridiag_dc.hpp), eigenvectors
@@ -477,15 +560,17 @@ __device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, in
 template <typename R, int NMAX>
 __device__ __noinline__ void dc_phase(int n, R* Z, int ld, R* work, unsigned long long* prof) {
   EigShared<R, NMAX>& sh = eig_sh<R, NMAX>;
-  WgTeam tm{static_cast<int>(threadIdx.x), prof};
-  dc_tridiag<R, NMAX, EigCfg<NMAX>::NT>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, Z, ld, work,
-                                        work + (size_t)n * n, sh.dc, sh.dct);
+  n = uniform(n);
+  WgTeam tm{static_cast<int>(threadIdx.x), uniform(prof)};
+  BIPP_GLOBAL R* w = global_ptr(work);
+  dc_tridiag<R, NMAX, EigCfg<NMAX>::NT>(tm, n, &sh.de[0].x, &sh.de[0].y, 2, global_ptr(Z),
+                                        uniform(ld), w, w + (size_t)n * n, sh.dc, sh.dct);
 }
 
 // records the wall clock at phase boundaries when the caller asked for a profile
 template <typename R>
 __device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) {
-  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * 16 + k] = wall_clock64();
+  if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * EIG_PROF_SLOTS + k] = wall_clock64();
 }
 
 template <typename R, typename TIn, int NMAX>
@@ -601,14 +686,14 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
     }
     __syncthreads();
   } else {
-    tridiag_phase<R, NMAX>(n, A, ld);
+    tridiag_phase<R, NMAX>(n, A, ld, p.prof ? p.prof + (size_t)sid * EIG_PROF_SLOTS + 16 : nullptr);
   }
   eig_mark(p, sid, 4);
 
   // ---- eigenpairs of the real tridiagonal by divide and conquer, eigenvectors into Z ----
   // (X, free until the back-transform, holds the two n x n real work arrays)
   dc_phase<R, NMAX>(n, Z, ld, reinterpret_cast<R*>(X),
-                    p.prof ? p.prof + (size_t)sid * 16 + 8 : nullptr);
+                    p.prof ? p.prof + (size_t)sid * EIG_PROF_SLOTS + 8 : nullptr);
   eig_mark(p, sid, 5);
 
   // ---- ascending order: rank sort --------------------------------------------------

@@ -50,9 +50,11 @@ struct EigParams {
   long long ws_stride = 0;
   int ws_ld = 0;
   int* info = nullptr;
-  // optional phase timestamps (wall clock ticks), [n_jobs][8] by sample index
+  // optional profile (wall clock ticks), EIG_PROF_SLOTS per sample: [0, 8) phase-boundary
+  // timestamps, [8, 16) divide-and-conquer step sums, [16, 20) tridiagonalisation step sums
   unsigned long long* prof = nullptr;
 };
+constexpr int EIG_PROF_SLOTS = 24;
 
 template <typename R>
 struct UnbeamParams {

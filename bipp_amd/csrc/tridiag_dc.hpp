@@ -36,6 +36,9 @@
 #ifndef BIPP_HD
 #define BIPP_HD __host__ __device__
 #endif
+#ifndef BIPP_GLOBAL
+#define BIPP_GLOBAL  // pointee qualifier of the work arrays (the device build: global memory)
+#endif
 
 #include <limits>
 
@@ -252,8 +255,9 @@ BIPP_HD int dc_count(const R* d, int st, int b, int e, R x, bool strict) {
 // n1 is a multiple of TR), k in chunks of TK staged through LDS, the next chunk's loads in
 // flight while the current one is used.  Each thread accumulates an MR x MC block.
 template <typename R, int NMAX, int NT, class Team>
-BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const R* Bp, int ldp, R* Bn,
-                              int ldn, const R* Mb, const int* slot_of, R* tile) {
+BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBAL R* Bp, int ldp,
+                              BIPP_GLOBAL R* Bn, int ldn, const BIPP_GLOBAL R* Mb,
+                              const int* slot_of, R* tile) {
   constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
   constexpr int MR = TR * TR / NT < 4 ? TR * TR / NT : 4;
   constexpr int MC = TR * TR / (NT * MR);
@@ -321,8 +325,8 @@ BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const R* Bp, int
 // tile holds DcCfg<NMAX>::TILE_ELEMS reals of fast memory.  Requires 1 <= n <= NMAX and a
 // team of NT threads.
 template <typename R, int NMAX, int NT, class Team>
-BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W, R* M,
-                        DcShared<R, NMAX>& sh, R* tile) {
+BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, BIPP_GLOBAL R* Z, int ldz,
+                        BIPP_GLOBAL R* W, BIPP_GLOBAL R* M, DcShared<R, NMAX>& sh, R* tile) {
   constexpr int TR = DcCfg<NMAX>::TR;
   constexpr int MAXIT = DcCfg<NMAX>::MAXIT;
   // lanes per secular root: a level has at most n <= NMAX roots
@@ -358,7 +362,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
   }
   tm.sync();
   {
-    R* B0 = buf(0);
+    BIPP_GLOBAL R* B0 = buf(0);
     const int ld0 = bld(0);
     for (int i = tid; i < n; i += NT) {
       R v = d[(long long)i * st];
@@ -376,9 +380,9 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
   for (int l = 1; l <= L; ++l) {
     const int h = 1 << (l - 1), S = 1 << l;
     const int nm = (n + S - 1) >> l;
-    const R* Bp = buf(l - 1);
+    const BIPP_GLOBAL R* Bp = buf(l - 1);
     const int ldp = bld(l - 1);
-    R* Bn = buf(l);
+    BIPP_GLOBAL R* Bn = buf(l);
     const int ldn = bld(l);
 
     // ---- 1. z and the merged order of the poles ----------------------------------------
@@ -572,7 +576,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
       if (r0 + h >= n) continue;
       const int sz = n - r0 < S ? n - r0 : S;
       const int c = g - r0;  // one column per thread
-      R* Mb = M + (long long)r0 * S;
+      BIPP_GLOBAL R* Mb = M + (long long)r0 * S;
       for (int t = sh.mR[m] - 1; t >= 0; --t) {
         const int a = sh.rp[r0 + t], b = sh.rn[r0 + t];
         const R cc = sh.rc[r0 + t], ss = sh.rs[r0 + t];
@@ -613,7 +617,7 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, R* Z, int ldz, R* W
         const int sz = n - r0 < S ? n - r0 : S;
         if (c >= sz) continue;
         const int k0 = row - r0 < h ? 0 : h, k1 = row - r0 < h ? h : sz;
-        const R* Mb = M + (long long)r0 * S;
+        const BIPP_GLOBAL R* Mb = M + (long long)r0 * S;
         R acc = 0;
         for (int k = k0; k < k1; ++k)
           acc = fma(Bp[row + (long long)(r0 + k) * ldp],
