@@ -24,6 +24,7 @@
 #include "tridiag_dc.hpp"
 
 #include <limits>
+#include <type_traits>
 
 namespace bippamd {
 
@@ -144,9 +145,13 @@ __shared__ EigShared<R, NMAX> eig_sh;
 // and below the diagonal).  fa(i, k) and fb(k, j) return operand elements, zero outside
 // their nonzero pattern; krange(i0, j0, klo, khi) bounds the k that can contribute to a
 // tile.  A_KFAST / B_KFAST: consecutive k are contiguous in memory (else consecutive i / j),
-// which sets how the threads split a tile load.  Lanes own tile rows, wavefronts GT/NW
-// tile columns; the loads of the next k chunk are in flight while the current one is used.
-// Every thread executes the same sequence of barriers.
+// which sets how the threads split a tile load.  The loads of the next k chunk are in flight
+// while the current one is used.  Every thread executes the same sequence of barriers.
+// float64: the tile is 4 x 4 blocks of 16 x 16, 16 / NW blocks per wavefront, each complex
+// product as four v_mfma_f64_16x16x4f64 (re re, -im im into the real part, re im, im re into
+// the imaginary part); operand layouts: A[row l % 16][k l / 16], B[k l / 16][col l % 16],
+// result register r at row l / 16 + 4 r, column l % 16 (l: lane).  float32: lanes own tile
+// rows, wavefronts GT / NW tile columns, on the vector ALU.
 // --------------------------------------------------------------------------------------
 template <typename R, int NMAX, bool A_KFAST, bool B_KFAST, bool LOWER_ONLY, typename FA,
           typename FB, typename KR>
@@ -160,14 +165,24 @@ __device__ __forceinline__ void gemm_tiles(int n, FA fa, FB fb, KR krange,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   GemmTiles<R>& t = sh.gt;
   const int nt = (n + GT - 1) / GT;
+  constexpr bool MF = std::is_same<R, double>::value;
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  constexpr int BPW = 16 / NW_;             // 16 x 16 blocks per wavefront (float64)
+  const int br = wave & 3, bc0 = (wave >> 2) * BPW;
   for (int tj = 0; tj < nt; ++tj) {
     for (int ti = LOWER_ONLY ? tj : 0; ti < nt; ++ti) {
       const int i0 = ti * GT, j0 = tj * GT;
       int klo = 0, khi = 0;
       krange(i0, j0, klo, khi);
       cplx<R> acc[CPW];
+      f64x4 cr[BPW], ci[BPW];
 #pragma unroll
       for (int c = 0; c < CPW; ++c) acc[c] = mk<R>(0, 0);
+#pragma unroll
+      for (int q = 0; q < BPW; ++q) {
+        cr[q] = f64x4{0, 0, 0, 0};
+        ci[q] = f64x4{0, 0, 0, 0};
+      }
       cplx<R> ra[PER], rb[PER];
       auto fetch = [&](int k0) {
 #pragma unroll
@@ -191,19 +206,47 @@ __device__ __forceinline__ void gemm_tiles(int n, FA fa, FB fb, KR krange,
         }
         __syncthreads();
         if (k0 + GK < khi) fetch(k0 + GK);
-#pragma unroll 4
-        for (int k = 0; k < GK; ++k) {
-          const cplx<R> a = t.a[k][lane];
+        if constexpr (MF) {
 #pragma unroll
-          for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a * t.b[k][wave * CPW + c];
+          for (int ks = 0; ks < GK / 4; ++ks) {
+            const int kr = 4 * ks + (lane >> 4);
+            const cplx<R> av = t.a[kr][br * 16 + (lane & 15)];
+#pragma unroll
+            for (int q = 0; q < BPW; ++q) {
+              const cplx<R> bv = t.b[kr][(bc0 + q) * 16 + (lane & 15)];
+              cr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.re, cr[q], 0, 0, 0);
+              cr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.im, bv.im, cr[q], 0, 0, 0);
+              ci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.im, ci[q], 0, 0, 0);
+              ci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, bv.re, ci[q], 0, 0, 0);
+            }
+          }
+        } else {
+#pragma unroll 4
+          for (int k = 0; k < GK; ++k) {
+            const cplx<R> a = t.a[k][lane];
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a * t.b[k][wave * CPW + c];
+          }
         }
         __syncthreads();
       }
-      const int i = i0 + lane;
+      if constexpr (MF) {
 #pragma unroll
-      for (int c = 0; c < CPW; ++c) {
-        const int j = j0 + wave * CPW + c;
-        if (i < n && j < n) C[i + (size_t)j * ldc] = acc[c];
+        for (int q = 0; q < BPW; ++q) {
+          const int j = j0 + (bc0 + q) * 16 + (lane & 15);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + br * 16 + (lane >> 4) + 4 * r;
+            if (i < n && j < n) C[i + (size_t)j * ldc] = mk<R>(cr[q][r], ci[q][r]);
+          }
+        }
+      } else {
+        const int i = i0 + lane;
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+          const int j = j0 + wave * CPW + c;
+          if (i < n && j < n) C[i + (size_t)j * ldc] = acc[c];
+        }
       }
     }
   }
