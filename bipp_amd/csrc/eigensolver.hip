@@ -91,6 +91,7 @@ struct BtCfg {
   static constexpr int RR = NMAX / LPC;
   static constexpr int CPWV = 64 / LPC;
   static constexpr int RB = NMAX >= 512 ? 4 : 8;
+  static constexpr int CPG = NMAX >= 512 ? 1 : 2;  // columns per lane group
 };
 
 template <typename R, int NMAX>
@@ -756,24 +757,29 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.de[sh.perm[j]].x : R(0);
 
   // ---- back-transform X = Q Z[:, perm] ------------------------------------------------
-  // Each group of LPC lanes keeps one column of Z[:, perm] in registers (rows g + LPC r)
-  // while H_{n-2}, ..., H_0 are applied to it, the reflectors staged through LDS RB at a
+  // Each group of LPC lanes keeps CPG columns of Z[:, perm] in registers (rows g + LPC r)
+  // while H_{n-2}, ..., H_0 are applied to them (the columns' dependent chains of dot
+  // product, reduction and update interleave), the reflectors staged through LDS RB at a
   // time; row blocks that lie above a reflector's unit entry are skipped.
   {
     constexpr int LPC = BtCfg<NMAX>::LPC, RR = BtCfg<NMAX>::RR, CPWV = BtCfg<NMAX>::CPWV,
-                  BT_RB = BtCfg<NMAX>::RB, RG = RR >= 4 ? 4 : RR;
+                  BT_RB = BtCfg<NMAX>::RB, RG = RR >= 4 ? 4 : RR, CPG = BtCfg<NMAX>::CPG;
     static_assert(LPC == 16 || LPC == 32, "column groups of 16 or 32 lanes");
     BackTile<R, NMAX>& bt = sh.bt;
     const int g = lane % LPC;
-    for (int jb = 0; jb < n; jb += NW_ * CPWV) {
-      const int j = jb + wave * CPWV + lane / LPC;
-      const bool live = j < n;
-      const int pj = live ? sh.perm[j] : 0;
-      cplx<R> x[RR];
+    for (int jb = 0; jb < n; jb += NW_ * CPWV * CPG) {
+      int jc[CPG];
+      cplx<R> x[CPG][RR];
 #pragma unroll
-      for (int r = 0; r < RR; ++r) {
-        const int i = g + LPC * r;
-        x[r] = mk<R>(live && i < n ? Z[i + (size_t)pj * ld] : R(0), R(0));
+      for (int u = 0; u < CPG; ++u) {
+        jc[u] = jb + (wave * CPWV + lane / LPC) * CPG + u;
+        const bool live = jc[u] < n;
+        const int pj = live ? sh.perm[jc[u]] : 0;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+          const int i = g + LPC * r;
+          x[u][r] = mk<R>(live && i < n ? Z[i + (size_t)pj * ld] : R(0), R(0));
+        }
       }
       // the chunk of reflectors cb, cb - 1, ... is fetched into registers one chunk ahead
       constexpr int PER = BT_RB * NMAX / NT_;
@@ -815,41 +821,56 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
           // row blocks r < RG g0 hold rows <= c only, where v = 0: skipped RG blocks at a
           // time (a uniform branch per group, the group's LDS loads issued together)
           const int g0 = (c + 1) / (LPC * RG);
-          cplx<R> dt = mk<R>(0, 0);
+          cplx<R> dt[CPG];
+#pragma unroll
+          for (int u = 0; u < CPG; ++u) dt[u] = mk<R>(0, 0);
 #pragma unroll
           for (int gq = 0; gq < RR / RG; ++gq) {
             if (gq >= g0) {
               cplx<R> vv[RG];
 #pragma unroll
-              for (int u = 0; u < RG; ++u) vv[u] = bt.v[q][g + LPC * (gq * RG + u)];
+              for (int w = 0; w < RG; ++w) vv[w] = bt.v[q][g + LPC * (gq * RG + w)];
 #pragma unroll
-              for (int u = 0; u < RG; ++u) dt = dt + cmul_conj_a(vv[u], x[gq * RG + u]);
+              for (int w = 0; w < RG; ++w)
+#pragma unroll
+                for (int u = 0; u < CPG; ++u)
+                  dt[u] = dt[u] + cmul_conj_a(vv[w], x[u][gq * RG + w]);
             }
           }
-          dt.re = sum16(dt.re);
-          dt.im = sum16(dt.im);
-          if (LPC == 32) {
-            dt.re += __shfl_xor(dt.re, 16, 64);
-            dt.im += __shfl_xor(dt.im, 16, 64);
+          cplx<R> t[CPG];
+#pragma unroll
+          for (int u = 0; u < CPG; ++u) {
+            dt[u].re = sum16(dt[u].re);
+            dt[u].im = sum16(dt[u].im);
+            if (LPC == 32) {
+              dt[u].re += __shfl_xor(dt[u].re, 16, 64);
+              dt[u].im += __shfl_xor(dt[u].im, 16, 64);
+            }
+            t[u] = tau * dt[u];
           }
-          const cplx<R> t = tau * dt;
 #pragma unroll
           for (int gq = 0; gq < RR / RG; ++gq) {
             if (gq >= g0) {
               cplx<R> vv[RG];
 #pragma unroll
-              for (int u = 0; u < RG; ++u) vv[u] = bt.v[q][g + LPC * (gq * RG + u)];
+              for (int w = 0; w < RG; ++w) vv[w] = bt.v[q][g + LPC * (gq * RG + w)];
 #pragma unroll
-              for (int u = 0; u < RG; ++u) x[gq * RG + u] = x[gq * RG + u] - vv[u] * t;
+              for (int w = 0; w < RG; ++w)
+#pragma unroll
+                for (int u = 0; u < CPG; ++u)
+                  x[u][gq * RG + w] = x[u][gq * RG + w] - vv[w] * t[u];
             }
           }
         }
       }
-      if (live) {
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-          const int i = g + LPC * r;
-          if (i < n) X[i + (size_t)j * ldx] = x[r];
+      for (int u = 0; u < CPG; ++u) {
+        if (jc[u] < n) {
+#pragma unroll
+          for (int r = 0; r < RR; ++r) {
+            const int i = g + LPC * r;
+            if (i < n) X[i + (size_t)jc[u] * ldx] = x[u][r];
+          }
         }
       }
     }
