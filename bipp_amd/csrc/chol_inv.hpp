@@ -56,31 +56,57 @@ struct CiShared {
   };
 };
 
+// Tile arithmetic of ci_gemm on the vector units: thread t accumulates tile row t mod CI_T
+// and CPW consecutive columns of group t / CI_T.  A team may bring its own (Team::CiTile,
+// the same members), e.g. one on matrix instructions.
+template <typename R, int NT>
+struct CiValuTile {
+  static constexpr int CPW = CI_T * CI_T / NT;
+  static_assert(CPW * (NT / CI_T) == CI_T, "tile splits over NT");
+  cplx<R> acc[CPW];
+  BIPP_HD void init() {
+    for (int c = 0; c < CPW; ++c) acc[c] = mk<R>(0, 0);
+  }
+  // acc += the staged chunk's product (sh.gt.a: [k][row], sh.gt.b: [k][column])
+  BIPP_HD void step(const CiShared<R>& sh, int tid) {
+    const int row = tid % CI_T, cg = tid / CI_T;
+    for (int k = 0; k < CI_K; ++k) {
+      const cplx<R> a = sh.gt.a[k][row];
+      for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a * sh.gt.b[k][cg * CPW + c];
+    }
+  }
+  template <class ST>
+  BIPP_HD void store(int i0, int j0, int m, int nn, ST st, int tid) const {
+    const int i = i0 + tid % CI_T;
+    for (int c = 0; c < CPW; ++c) {
+      const int j = j0 + (tid / CI_T) * CPW + c;
+      if (i < m && j < nn) st(i, j, acc[c]);
+    }
+  }
+};
+
 // C = A B over rows [0, m) and columns [0, nn) in CI_T x CI_T tiles (LOWER: only tiles with
 // ti >= tj).  fa(i, k) and fb(k, j) give the operands (zero outside their pattern);
 // krange(i0, j0, klo, khi) bounds the k of a tile; store(i, j, acc) writes a result.
 // B_KFAST: consecutive k of B are contiguous in memory (else consecutive j), which sets how
-// the threads split a tile load.  Thread t owns tile row t mod 64 and CPW consecutive
-// columns of group t / 64; the next k chunk is loaded into registers while the current one
-// is used.  Ends with a barrier (the stores are visible to the next product).
+// the threads split a tile load.  The arithmetic is the team's tile type (Team::CiTile);
+// the next k chunk is loaded into registers while the current one is used.  Ends with a
+// barrier (the stores are visible to the next product).
 template <typename R, int NT, bool LOWER, bool B_KFAST, class Team, class FA, class FB, class KR,
           class ST>
 BIPP_HD void ci_gemm(Team& tm, int m, int nn, FA fa, FB fb, KR krange, ST store,
                      CiShared<R>& sh) {
-  constexpr int CPW = CI_T * CI_T / NT;
   constexpr int PER = CI_T * CI_K / NT;
-  static_assert(CPW * (NT / CI_T) == CI_T && PER * NT == CI_T * CI_K, "tile splits over NT");
+  static_assert(PER * NT == CI_T * CI_K, "tile loads split over NT");
   const int tid = tm.tid;
-  const int row = tid % CI_T, cg = tid / CI_T;
   const int ntr = (m + CI_T - 1) / CI_T, ntc = (nn + CI_T - 1) / CI_T;
   for (int tj = 0; tj < ntc; ++tj) {
     for (int ti = LOWER ? tj : 0; ti < ntr; ++ti) {
       const int i0 = ti * CI_T, j0 = tj * CI_T;
       int klo = 0, khi = 0;
       krange(i0, j0, klo, khi);
-      cplx<R> acc[CPW];
-#pragma unroll
-      for (int c = 0; c < CPW; ++c) acc[c] = mk<R>(0, 0);
+      typename Team::template CiTile<R, NT> acc;
+      acc.init();
       cplx<R> ra[PER], rb[PER];
       auto fetch = [&](int k0) {
 #pragma unroll
@@ -106,19 +132,10 @@ BIPP_HD void ci_gemm(Team& tm, int m, int nn, FA fa, FB fb, KR krange, ST store,
         }
         tm.sync();
         if (k0 + CI_K < khi) fetch(k0 + CI_K);
-        for (int k = 0; k < CI_K; ++k) {
-          const cplx<R> a = sh.gt.a[k][row];
-#pragma unroll
-          for (int c = 0; c < CPW; ++c) acc[c] = acc[c] + a * sh.gt.b[k][cg * CPW + c];
-        }
+        acc.step(sh, tid);
         tm.sync();
       }
-      const int i = i0 + row;
-#pragma unroll
-      for (int c = 0; c < CPW; ++c) {
-        const int j = j0 + cg * CPW + c;
-        if (i < m && j < nn) store(i, j, acc[c]);
-      }
+      acc.store(i0, j0, m, nn, store, tid);
     }
   }
   tm.sync();

@@ -508,11 +508,61 @@ __device__ __forceinline__ BIPP_GLOBAL T* global_ptr(T* p) {
   return (BIPP_GLOBAL T*)uniform(p);
 }
 
+// ci_gemm's tile arithmetic in float64 on v_mfma_f64_16x16x4f64: the 64 x 64 tile as 4 x 4
+// blocks of 16 x 16, 16 / NW blocks per wavefront, each complex product as four MFMAs (the
+// operand and result layouts of gemm_tiles).
+template <int NT>
+struct CiMfmaTile {
+  static constexpr int NW = NT / 64, BPW = 16 / NW;
+  static_assert(CI_T == 64 && CI_K % 4 == 0 && BPW * NW == 16, "4 x 4 blocks of 16");
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  f64x4 cr[BPW], ci[BPW];
+  __device__ void init() {
+#pragma unroll
+    for (int q = 0; q < BPW; ++q) {
+      cr[q] = f64x4{0, 0, 0, 0};
+      ci[q] = f64x4{0, 0, 0, 0};
+    }
+  }
+  __device__ void step(const CiShared<double>& sh, int tid) {
+    const int lane = tid & 63, wave = tid >> 6, br = wave & 3, bc0 = (wave >> 2) * BPW;
+#pragma unroll
+    for (int ks = 0; ks < CI_K / 4; ++ks) {
+      const int kr = 4 * ks + (lane >> 4);
+      const cplx<double> av = sh.gt.a[kr][br * 16 + (lane & 15)];
+#pragma unroll
+      for (int q = 0; q < BPW; ++q) {
+        const cplx<double> bv = sh.gt.b[kr][(bc0 + q) * 16 + (lane & 15)];
+        cr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.re, cr[q], 0, 0, 0);
+        cr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.im, bv.im, cr[q], 0, 0, 0);
+        ci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.im, ci[q], 0, 0, 0);
+        ci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, bv.re, ci[q], 0, 0, 0);
+      }
+    }
+  }
+  template <class ST>
+  __device__ void store(int i0, int j0, int m, int nn, ST st, int tid) const {
+    const int lane = tid & 63, wave = tid >> 6, br = wave & 3, bc0 = (wave >> 2) * BPW;
+#pragma unroll
+    for (int q = 0; q < BPW; ++q) {
+      const int j = j0 + (bc0 + q) * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + br * 16 + (lane >> 4) + 4 * r;
+        if (i < m && j < nn) st(i, j, mk<double>(cr[q][r], ci[q][r]));
+      }
+    }
+  }
+};
+
 // The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
 struct WgTeam {
   int tid;
   unsigned long long* prof = nullptr;  // step k's wall-clock time added to prof[k] (thread 0)
   unsigned long long t = 0;
+  template <typename R, int NT>
+  using CiTile =
+      typename std::conditional<std::is_same<R, double>::value, CiMfmaTile<NT>, CiValuTile<R, NT>>::type;
   __device__ void sync() const { __syncthreads(); }
   // total over the aligned group of P lanes (a + b == b + a: the same bits in each lane)
   template <int P, typename R>

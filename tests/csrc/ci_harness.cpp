@@ -23,6 +23,8 @@ struct CpuTeam {
   int tid;
   std::barrier<>* bar;
   void sync() { bar->arrive_and_wait(); }
+  template <typename R, int NT>
+  using CiTile = bippamd::CiValuTile<R, NT>;
 };
 
 template <typename R, int NT>
