@@ -380,7 +380,7 @@ __device__ __forceinline__ cplx<R> reflector(int n, int c, const BufMat<R>& M, c
 //   conj(A[i, j]) vn_i to x_j (a per-column reduction over the wavefront's lanes).
 // Result x[i0:n) is written to sh.w (v and w are no longer needed afterwards).
 template <typename R, int NMAX, bool APPLY>
-__device__ __forceinline__ void trail_pass(int n, int i0, cplx<R>* __restrict__ A, int ld,
+__device__ __forceinline__ void trail_pass(int n, int i0, BIPP_GLOBAL cplx<R>* __restrict__ A, int ld,
                                            const cplx<R>* v, const cplx<R>* vn,
                                            EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
@@ -400,11 +400,11 @@ __device__ __forceinline__ void trail_pass(int n, int i0, cplx<R>* __restrict__ 
 #pragma unroll
     for (int u = 0; u < UC; ++u) {
       const int j = jb + u * NW_;
-      const cplx<R>* col = A + (size_t)j * ld;
+      const BIPP_GLOBAL cplx<R>* col = A + (size_t)j * ld;
 #pragma unroll
       for (int r = 0; r < RM; ++r) {
         const int i = i0 + lane + 64 * r;
-        if (j < n && i < n && i >= j) a[u][r] = col[i];
+        if (j < n && i < n && i >= j) a[u][r] = ci_get(col[i]);
       }
     }
     R pre[UC], pim[UC];
@@ -420,7 +420,7 @@ __device__ __forceinline__ void trail_pass(int n, int i0, cplx<R>* __restrict__ 
         cwj = conj(sh.w[j]);
       }
       const cplx<R> vnj = vn[j];
-      cplx<R>* col = A + (size_t)j * ld;
+      BIPP_GLOBAL cplx<R>* col = A + (size_t)j * ld;
 #pragma unroll
       for (int r = 0; r < RM; ++r) {
         const int i = i0 + lane + 64 * r;
@@ -428,7 +428,7 @@ __device__ __forceinline__ void trail_pass(int n, int i0, cplx<R>* __restrict__ 
           cplx<R> x = a[u][r];
           if (APPLY) {
             x = x - (vi[r] * cwj + wi[r] * cvj);
-            col[i] = x;
+            ci_put(col[i], x);
           }
           acc[r] = acc[r] + x * vnj;
           if (i > j) {
@@ -605,7 +605,7 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
     sh.de[0].x = d0;
     M.store(M.off(0, 0), tau);  // diagonal slot keeps tau_0 for the back-transform
   }
-  trail_pass<R, NMAX, false>(n, 1, A, ld, vcur, vcur, sh);
+  trail_pass<R, NMAX, false>(n, 1, (BIPP_GLOBAL cplx<R>*)A, ld, vcur, vcur, sh);
   make_w<R, NMAX>(n, 0, tau, vcur, sh);
   for (int k = 0; k + 2 < n; ++k) {
     // pending update (v_k, w_k) applied to column k+1, rows [k+1, n): row k + 1 + tid
@@ -622,7 +622,7 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
     tau = reflector<R, NMAX>(n, k + 1, M, vnext, sh);
     if (tid == 0) M.store(M.off(k + 1, k + 1), tau);
     tm.mark(0);
-    trail_pass<R, NMAX, true>(n, k + 2, A, ld, vcur, vnext, sh);
+    trail_pass<R, NMAX, true>(n, k + 2, (BIPP_GLOBAL cplx<R>*)A, ld, vcur, vnext, sh);
     tm.mark(1);
     make_w<R, NMAX>(n, k + 1, tau, vnext, sh);
     tm.mark(2);
