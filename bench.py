@@ -382,8 +382,8 @@ def main() -> None:
         achieved = nufft_bytes / (kern_ms * 1e-3) / 1e9
         stage_rl["nufft"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                              "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                             "kernel": "n3_points/scatter, n3_spread_kernel, fft_lines/fft_pass_rr "
-                                       "(pruned), n3_interp_kernel",
+                             "kernel": "n3_points_kernel (two passes), n3_spread_kernel, "
+                                       "fft_lines/fft_pass_rr (pruned), n3_interp_kernel",
                              "kernel_ms": kern_ms, "fine_grid": list(nf), "spread_grid": nf1,
                              "points": m_pts, "algorithmic_bytes_per_launch": nufft_bytes}
         # measured HBM bytes of the chain: per-dispatch counters times dispatches per step
