@@ -419,45 +419,55 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, BIPP_GLOBAL R* Z, i
         zmax = b > zmax ? b : zmax;
       }
       const R tol = R(8) * eps * (dmax > zmax ? dmax : zmax);
+      // the pending pole pj's (d, z) stay in registers; the rotation test |t c s| <= tol
+      // is taken as |t zp zn| <= tol (zp^2 + zn^2) (c s = -zn zp / th^2), so the square root
+      // and the divisions are only evaluated for a rotation
       int K = 0, nr = 0, pj = -1;
+      R dj = 0, zj = 0;
       for (int p = r0; p < r0 + sz; ++p) {
-        if (rho * dc_abs(sh.zs[p]) <= tol) {
+        const R zn = sh.zs[p], dn = sh.ds[p];
+        if (rho * dc_abs(zn) <= tol) {
           sh.kind[p] = -1;
           continue;
         }
         if (pj < 0) {
           pj = p;
+          dj = dn;
+          zj = zn;
           continue;
         }
-        const R zp = sh.zs[pj], zn = sh.zs[p];
-        const R th = sqrt(zp * zp + zn * zn);
-        const R c = zn / th, s = -zp / th;
-        const R t = sh.ds[p] - sh.ds[pj];
-        if (dc_abs(t * c * s) <= tol) {
+        const R t = dn - dj;
+        const R th2 = zj * zj + zn * zn;
+        if (dc_abs(t * zj * zn) <= tol * th2) {
           // rotate slot pj's z into slot p: pj keeps an eigenvalue of its own
-          sh.zs[p] = th;
+          const R th = sqrt(th2);
+          const R c = zn / th, s = -zj / th;
           sh.zs[pj] = R(0);
           sh.rp[r0 + nr] = pj - r0;
           sh.rn[r0 + nr] = p - r0;
           sh.rc[r0 + nr] = c;
           sh.rs[r0 + nr] = s;
           ++nr;
-          const R dpj = sh.ds[pj], dp = sh.ds[p];
-          sh.ds[pj] = dpj * c * c + dp * s * s;
-          sh.ds[p] = dpj * s * s + dp * c * c;
+          sh.ds[pj] = dj * c * c + dn * s * s;
           sh.kind[pj] = -1;
+          dj = dj * s * s + dn * c * c;  // slot p's pole and z after the rotation
+          zj = th;
+          sh.ds[p] = dj;
+          sh.zs[p] = zj;
         } else {
           sh.kind[pj] = K;
-          sh.dk[r0 + K] = sh.ds[pj];
-          sh.zk[r0 + K] = sh.zs[pj];
+          sh.dk[r0 + K] = dj;
+          sh.zk[r0 + K] = zj;
           ++K;
+          dj = dn;
+          zj = zn;
         }
         pj = p;
       }
       if (pj >= 0) {
         sh.kind[pj] = K;
-        sh.dk[r0 + K] = sh.ds[pj];
-        sh.zk[r0 + K] = sh.zs[pj];
+        sh.dk[r0 + K] = dj;
+        sh.zk[r0 + K] = zj;
         ++K;
       }
       sh.mK[m] = K;
