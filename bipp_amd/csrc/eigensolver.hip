@@ -94,6 +94,19 @@ struct BtCfg {
   static constexpr int CPG = NMAX >= 512 ? 1 : 2;  // columns per lane group
 };
 
+// The float64 back-transform on matrix instructions (n <= NMAX <= 256): one block of 16
+// reflectors over all rows, the wavefronts' partial products, the block's Gram matrix, T
+// factor and tau.
+constexpr bool bt_mfma(int nmax) { return nmax <= 256; }
+template <typename R, int NMAX>
+struct BtBlock {
+  cplx<R> v[16][NMAX + 1];            // v[q][i]: reflector q of the block at row i
+  R red[EigCfg<NMAX>::NW][64][8];     // per wavefront: a 16 x 16 complex result, lane layout
+  cplx<R> g[16][16];                  // V^H V
+  cplx<R> t[16][16];                  // T (upper triangular)
+  cplx<R> tau[16];
+};
+
 template <typename R, int NMAX>
 struct BackTile {
   cplx<R> v[BtCfg<NMAX>::RB][NMAX];  // reflector vectors (zero above the unit entry)
@@ -122,6 +135,8 @@ struct EigShared {
     cplx<R> xw[NW_][NMAX];  // per-wave partial sums of the matrix-vector passes
     GemmTiles<R> gt;        // operand tiles of the GEMM passes
     BackTile<R, NMAX> bt;   // reflectors of the back-transform
+    typename std::conditional<std::is_same<R, double>::value && bt_mfma(NMAX), BtBlock<R, NMAX>,
+                              char>::type bb;  // blocks of the back-transform on MFMA
     DcShared<R, NMAX> dc;   // per-merge arrays of the tridiagonal divide and conquer
     CiShared<R> ci;         // diagonal blocks and product tiles of the Cholesky inverse
   };
@@ -667,6 +682,217 @@ __device__ __forceinline__ void eig_mark(const EigParams<R>& p, int sid, int k) 
   if (p.prof && threadIdx.x == 0) p.prof[(size_t)sid * EIG_PROF_SLOTS + k] = wall_clock64();
 }
 
+// X = Q Z[:, perm] for Q = H_0 H_1 ... H_{n-2} (H_c = I - tau_c v_c v_c^H, v_c below the
+// subdiagonal of column c of A, tau_c on its diagonal), float64 on v_mfma_f64_16x16x4f64.
+// The reflectors go in blocks of 16 from the last, H_cs ... H_c1 = I - V T V^H (compact WY,
+// T upper triangular: T_ii = tau_i, T[0:i, i] = -tau_i T[0:i, 0:i] (V^H V)[0:i, i]).  The T
+// of every block is formed first (V^H V on MFMA, the recurrence on 16 lanes) and kept in the
+// strict upper triangle of A, which nothing reads from the tridiagonalisation until the
+// un-whitening overwrites A.  Then panels of Z[:, perm] stay in registers while every block
+// is applied: X -= V (T (V^H X)).  A panel holds one 16-column block per wavefront pair;
+// the two wavefronts of a pair take alternate 16-row blocks, so V^H X needs one sum of the
+// pair's two partial products through LDS.  MFMA layouts (l: lane): A[l % 16][l / 16],
+// B[l / 16][l % 16], result register r at row l / 16 + 4 r, column l % 16; a result
+// block's register s is therefore the B operand for its rows 4 s .. 4 s + 3, and every
+// product chains in registers.  Row blocks above a block's first unit entry (where V is
+// zero) are skipped.
+template <int NMAX>
+__device__ __noinline__ void back_transform_mfma(int n, cplx<double>* A_, int ld,
+                                                 const double* Z_, cplx<double>* X_, int ldx) {
+  EigShared<double, NMAX>& sh = eig_sh<double, NMAX>;
+  n = uniform(n);
+  ld = uniform(ld);
+  ldx = uniform(ldx);
+  BIPP_GLOBAL cplx<double>* A = global_ptr(A_);
+  const BIPP_GLOBAL double* Z = global_ptr(Z_);
+  BIPP_GLOBAL cplx<double>* X = global_ptr(X_);
+  constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW;
+  constexpr int NRB = NMAX / 16, RBW = NRB / 2, PW = NW_ / 2 * 16;
+  static_assert(NRB % 2 == 0 && NW_ % 2 == 0, "row blocks and wavefronts in pairs");
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int half = wave & 1, cg = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  BtBlock<double, NMAX>& bb = sh.bb;
+  const int nblk = (n - 1 + 15) / 16;  // reflectors 0 .. n - 2
+  const f64x4 z4 = {0, 0, 0, 0};
+
+  // stage block b: bb.v[q] = v_{cs + q} (zero for cs + q < 0), bb.tau
+  auto stage = [&](int b) {
+    const int cs = n - 2 - 16 * b - 15;
+    for (int e = tid; e < 16 * NMAX; e += NT_) {
+      const int q = e / NMAX, i = e % NMAX, c = cs + q;
+      cplx<double> val = mk<double>(0, 0);
+      if (c >= 0 && i < n) {
+        if (i == c + 1)
+          val = mk<double>(1, 0);
+        else if (i > c + 1)
+          val = ci_get(A[i + (size_t)c * ld]);
+      }
+      bb.v[q][i] = val;
+    }
+    if (tid < 16)
+      bb.tau[tid] =
+          cs + tid >= 0 ? ci_get(A[(cs + tid) + (size_t)(cs + tid) * ld]) : mk<double>(0, 0);
+  };
+  // first row block holding a nonzero of block b's V
+  auto rb_first = [&](int b) {
+    const int cs = n - 2 - 16 * b - 15;
+    return ((cs < 0 ? 0 : cs) + 1) / 16;
+  };
+
+  // ---- T of every block, into the strict upper triangle of A --------------------------
+  for (int b = 0; b < nblk; ++b) {
+    const int cs = n - 2 - 16 * b - 15, rb0 = rb_first(b);
+    __syncthreads();  // the previous block's LDS is no longer read
+    stage(b);
+    __syncthreads();
+    f64x4 gr = z4, gi = z4;
+    for (int rb = wave; rb < NRB; rb += NW_) {
+      if (rb < rb0) continue;  // uniform over the wavefront
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const cplx<double> v = bb.v[lr][rb * 16 + 4 * s4 + lk];
+        gr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, v.re, gr, 0, 0, 0);
+        gr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, v.im, gr, 0, 0, 0);
+        gi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, v.im, gi, 0, 0, 0);
+        gi = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, v.re, gi, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      bb.red[wave][lane][r] = gr[r];
+      bb.red[wave][lane][4 + r] = gi[r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double sr = 0, si = 0;
+        for (int w = 0; w < NW_; ++w) {
+          sr += bb.red[w][lane][r];
+          si += bb.red[w][lane][4 + r];
+        }
+        bb.g[lk + 4 * r][lr] = mk<double>(sr, si);
+      }
+      wave_sync();
+      // lanes p < 16: column i of T from the columns before it
+      for (int i = 0; i < 16; ++i) {
+        if (lane < 16) {
+          const int pr = lane;
+          cplx<double> val = mk<double>(0, 0);
+          if (pr == i) {
+            val = bb.tau[i];
+          } else if (pr < i) {
+            cplx<double> acc = mk<double>(0, 0);
+            for (int q = pr; q < i; ++q) acc = acc + bb.t[pr][q] * bb.g[q][i];
+            val = mk<double>(0, 0) - bb.tau[i] * acc;
+          }
+          bb.t[pr][i] = val;
+        }
+        wave_sync();
+      }
+      // store T[p][q], p <= q, at A[cs + p, cs + q + 1] (strict upper triangle)
+      for (int e = lane; e < 256; e += 64) {
+        const int pr = e / 16, q = e % 16;
+        if (pr <= q && cs + pr >= 0) ci_put(A[(cs + pr) + (size_t)(cs + q + 1) * ld], bb.t[pr][q]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- panels ------------------------------------------------------------------------
+  for (int j0 = 0; j0 < n; j0 += PW) {
+    const int col = j0 + cg * 16 + lr;
+    const int pc = col < n ? sh.perm[col] : 0;
+    f64x4 xr[RBW], xi[RBW];
+#pragma unroll
+    for (int t = 0; t < RBW; ++t) {
+      const int rb = half + 2 * t;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + lk + 4 * r;
+        xr[t][r] = (row < n && col < n) ? Z[row + (size_t)pc * ld] : 0.0;
+      }
+      xi[t] = z4;
+    }
+    for (int b = 0; b < nblk; ++b) {
+      const int cs = n - 2 - 16 * b - 15, rb0 = rb_first(b);
+      __syncthreads();  // the previous block's LDS is no longer read
+      stage(b);
+      for (int e = tid; e < 256; e += NT_) {
+        const int pr = e / 16, q = e % 16;
+        bb.t[pr][q] = (pr <= q && cs + pr >= 0) ? ci_get(A[(cs + pr) + (size_t)(cs + q + 1) * ld])
+                                                : mk<double>(0, 0);
+      }
+      __syncthreads();
+      // W1 = V^H X over this wavefront's row blocks
+      f64x4 wr = z4, wi = z4;
+#pragma unroll
+      for (int t = 0; t < RBW; ++t) {
+        const int rb = half + 2 * t;
+        if (rb < rb0) continue;  // uniform over the wavefront
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const cplx<double> v = bb.v[lr][rb * 16 + 4 * s4 + lk];
+          wr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, xr[t][s4], wr, 0, 0, 0);
+          wr = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, xi[t][s4], wr, 0, 0, 0);
+          wi = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, xi[t][s4], wi, 0, 0, 0);
+          wi = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, xr[t][s4], wi, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        bb.red[wave][lane][r] = wr[r];
+        bb.red[wave][lane][4 + r] = wi[r];
+      }
+      __syncthreads();
+      // the pair's sum, in the same order in both wavefronts
+      f64x4 w1r, w1i;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        w1r[r] = bb.red[wave & ~1][lane][r] + bb.red[wave | 1][lane][r];
+        w1i[r] = bb.red[wave & ~1][lane][4 + r] + bb.red[wave | 1][lane][4 + r];
+      }
+      // W2 = T W1
+      f64x4 w2r = z4, w2i = z4;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const cplx<double> tv = bb.t[lr][4 * s4 + lk];
+        w2r = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.re, w1r[s4], w2r, 0, 0, 0);
+        w2r = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.im, w1i[s4], w2r, 0, 0, 0);
+        w2i = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.re, w1i[s4], w2i, 0, 0, 0);
+        w2i = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.im, w1r[s4], w2i, 0, 0, 0);
+      }
+      // X -= V W2
+#pragma unroll
+      for (int t = 0; t < RBW; ++t) {
+        const int rb = half + 2 * t;
+        if (rb < rb0) continue;  // uniform over the wavefront
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const cplx<double> v = bb.v[4 * s4 + lk][rb * 16 + lr];
+          xr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.re, w2r[s4], xr[t], 0, 0, 0);
+          xr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, w2i[s4], xr[t], 0, 0, 0);
+          xi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.re, w2i[s4], xi[t], 0, 0, 0);
+          xi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, w2r[s4], xi[t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < RBW; ++t) {
+      const int rb = half + 2 * t;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + lk + 4 * r;
+        if (row < n && col < n) ci_put(X[row + (size_t)col * ldx], mk<double>(xr[t][r], xi[t][r]));
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <typename R, typename TIn, int NMAX>
 __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh) {
   constexpr int NT_ = EigCfg<NMAX>::NT, NW_ = EigCfg<NMAX>::NW, RM = EigCfg<NMAX>::RM,
@@ -807,6 +1033,9 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   for (int j = tid; j < nmax; j += NT_) dout[j] = j < n ? sh.de[sh.perm[j]].x : R(0);
 
   // ---- back-transform X = Q Z[:, perm] ------------------------------------------------
+  if constexpr (std::is_same<R, double>::value && bt_mfma(NMAX)) {
+    back_transform_mfma<NMAX>(n, A, ld, Z, X, ldx);
+  } else
   // Each group of LPC lanes keeps CPG columns of Z[:, perm] in registers (rows g + LPC r)
   // while H_{n-2}, ..., H_0 are applied to them (the columns' dependent chains of dot
   // product, reduction and update interleave), the reflectors staged through LDS RB at a
