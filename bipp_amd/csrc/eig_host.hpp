@@ -190,9 +190,9 @@ struct EigRunner {
       std::fprintf(stderr, " | tridiag_eig:");
       for (int q = 0; q < 8; ++q) std::fprintf(stderr, " %s %.1f", subn[q], cnt ? sub[q] / cnt / 100.0 : 0.0);
       // steps of the tridiagonalisation, summed over its columns
-      static const char* tsn[3] = {"column and reflector", "pass", "w"};
+      static const char* tsn[2] = {"w, column update and reflector", "pass"};
       std::fprintf(stderr, " | tridiag:");
-      for (int q = 0; q < 3; ++q) std::fprintf(stderr, " %s %.1f", tsn[q], cnt ? tsub[q] / cnt / 100.0 : 0.0);
+      for (int q = 0; q < 2; ++q) std::fprintf(stderr, " %s %.1f", tsn[q], cnt ? tsub[q] / cnt / 100.0 : 0.0);
       std::fprintf(stderr, "\n");
     }
 

@@ -143,7 +143,7 @@ struct EigShared {
   RPair<R> de[NMAX];      // tridiagonal: (diagonal d_i, subdiagonal e_i)
   int perm[NMAX];
   int idx[NMAX];
-  R red[2 * NW_];
+  R red[2][2 * NW_];  // two slots: back-to-back reductions need no barrier between
   R scal[4];
   int iscal[4];
 };
@@ -280,24 +280,25 @@ __device__ __forceinline__ R wave_total(R v) {
 
 // Sums of a and b over the workgroup (every thread gets them), with one barrier: the
 // caller guarantees that another barrier separates this call from the next write of
-// sh.red (it holds 2 values per wavefront).
+// sh.red[slot] (it holds 2 values per wavefront).
 template <typename R, int NMAX>
-__device__ __forceinline__ void block_sum2(R& a, R& b, EigShared<R, NMAX>& sh) {
+__device__ __forceinline__ void block_sum2(R& a, R& b, EigShared<R, NMAX>& sh, int slot = 0) {
   constexpr int NW_ = EigCfg<NMAX>::NW;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  R* red = sh.red[slot];
   a = wave_total(a);
   b = wave_total(b);
   if (lane == 0) {
-    sh.red[wave] = a;
-    sh.red[NW_ + wave] = b;
+    red[wave] = a;
+    red[NW_ + wave] = b;
   }
   __syncthreads();
   a = 0;
   b = 0;
 #pragma unroll
   for (int w = 0; w < NW_; ++w) {
-    a += sh.red[w];
-    b += sh.red[NW_ + w];
+    a += red[w];
+    b += red[NW_ + w];
   }
 }
 
@@ -595,6 +596,71 @@ struct WgTeam {
   }
 };
 
+// After the pass for column k (sh.w holds y = A v_k): w_k = tau_k y - 1/2 tau_k (tau_k y)^H
+// v_k v_k into sh.w, the pending update (v_k, w_k) applied to column k + 1 and the reflector
+// of column k + 1 formed (v_{k+1} into vnext and below the subdiagonal of A, beta into
+// A[k + 2, k + 1] and sh.de[k + 1].y, the updated diagonal into sh.de[k + 1].x; returns
+// tau_{k+1}).  Thread t holds row k + 1 + t throughout, so the updated column stays in
+// registers (the reflector overwrites it in A); w_k at row k + 1 is formed by every thread
+// from its own copy of y_{k+1}; alpha of the reflector goes through LDS before the norm's
+// barrier.  Three barriers instead of the five of the separate steps.
+template <typename R, int NMAX>
+__device__ __forceinline__ cplx<R> post_column(int n, int k, cplx<R> tau, const BufMat<R>& M,
+                                               const cplx<R>* vcur, cplx<R>* vnext,
+                                               EigShared<R, NMAX>& sh) {
+  static_assert(EigCfg<NMAX>::NT >= NMAX, "one row per thread");
+  const int tid = threadIdx.x;
+  const int i = k + 1 + tid;
+  const bool live = i < n;
+  const unsigned oi = live ? M.off(i, k + 1) : BufMat<R>::OOB;
+  const cplx<R> a = M.load(oi);  // column k + 1, row i (in flight during the dot)
+  const cplx<R> yi = live ? sh.w[i] : mk<R>(0, 0);
+  const cplx<R> vi = live ? vcur[i] : mk<R>(0, 0);
+  const cplx<R> y1 = sh.w[k + 1], v1 = vcur[k + 1];  // row k + 1, for every thread
+  const cplx<R> xi = tau * yi;
+  const cplx<R> dt = cmul_conj_a(xi, vi);
+  R pr = dt.re, pi = dt.im;
+  block_sum2(pr, pi, sh, 0);
+  const cplx<R> alpha2 = tau * mk<R>(R(-0.5) * pr, R(-0.5) * pi);
+  const cplx<R> wi = xi + alpha2 * vi;
+  const cplx<R> w1 = tau * y1 + alpha2 * v1;
+  if (live) sh.w[i] = wi;
+  // column k + 1 with the update (v_k, w_k)
+  const cplx<R> c = a - (vi * conj(w1) + wi * conj(v1));
+  if (tid == 0) sh.de[k + 1].x = c.re;
+  // reflector of column k + 1: alpha = row k + 2 (thread 1), the rest below it
+  if (tid == 1) sh.scal[0] = c.re, sh.scal[1] = c.im;
+  R part = (tid >= 2 && live) ? abs2(c) : R(0), unused = 0;
+  block_sum2(part, unused, sh, 1);
+  const cplx<R> alpha = n > k + 2 ? mk<R>(sh.scal[0], sh.scal[1]) : mk<R>(0, 0);
+  const R xnorm2 = part;
+  cplx<R> tn, scal;
+  R beta;
+  if (xnorm2 == R(0) && alpha.im == R(0)) {
+    tn = mk<R>(0, 0);
+    beta = alpha.re;
+    scal = mk<R>(0, 0);
+  } else {
+    const R nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm2);
+    beta = alpha.re >= R(0) ? -nrm : nrm;
+    tn = mk<R>((beta - alpha.re) / beta, -alpha.im / beta);
+    scal = crecip(mk<R>(alpha.re - beta, alpha.im));
+  }
+  if (tid >= 2 && live) {
+    const cplx<R> v = c * scal;
+    M.store(oi, v);
+    vnext[i] = v;
+  }
+  if (tid == 1 && live) {
+    vnext[i] = mk<R>(1, 0);
+    M.store(oi, mk<R>(beta, 0));
+    sh.de[k + 1].y = beta;
+  }
+  if (tid == 0) M.store(M.off(k + 1, k + 1), tn);  // the diagonal slot keeps tau
+  __syncthreads();
+  return tn;
+}
+
 // Householder tridiagonalisation (zhetd2 order, lower triangle, n >= 2): d and e into
 // sh.de, the reflectors below the subdiagonal of A with tau in the diagonal slots (0 for the
 // last column).  Each column's pass applies the pending rank-2 update and forms the
@@ -612,7 +678,7 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
   cplx<R>* vcur = sh.v;
   cplx<R>* vnext = sh.vn;
   const BufMat<R> M(A, ld, n);
-  WgTeam tm{tid, prof};  // step timing: 0 column update and reflector, 1 pass, 2 w
+  WgTeam tm{tid, prof};  // step timing: 0 w, column update and reflector, 1 pass
   tm.mark(-1);
   const R d0 = M.load(M.off(0, 0)).re;
   cplx<R> tau = reflector<R, NMAX>(n, 0, M, vcur, sh);
@@ -621,30 +687,19 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
     M.store(M.off(0, 0), tau);  // diagonal slot keeps tau_0 for the back-transform
   }
   trail_pass<R, NMAX, false>(n, 1, (BIPP_GLOBAL cplx<R>*)A, ld, vcur, vcur, sh);
-  make_w<R, NMAX>(n, 0, tau, vcur, sh);
   for (int k = 0; k + 2 < n; ++k) {
-    // pending update (v_k, w_k) applied to column k+1, rows [k+1, n): row k + 1 + tid
-    {
-      const int i = k + 1 + tid;
-      if (i < n) {
-        const unsigned o = M.off(i, k + 1);
-        const cplx<R> x = M.load(o) - (vcur[i] * conj(sh.w[k + 1]) + sh.w[i] * conj(vcur[k + 1]));
-        M.store(o, x);
-        if (tid == 0) sh.de[k + 1].x = x.re;
-      }
-    }
-    __syncthreads();
-    tau = reflector<R, NMAX>(n, k + 1, M, vnext, sh);
-    if (tid == 0) M.store(M.off(k + 1, k + 1), tau);
+    // w_k, column k + 1 updated, reflector k + 1
+    tm.mark(1);
+    const cplx<R> tn = post_column<R, NMAX>(n, k, tau, M, vcur, vnext, sh);
     tm.mark(0);
     trail_pass<R, NMAX, true>(n, k + 2, (BIPP_GLOBAL cplx<R>*)A, ld, vcur, vnext, sh);
-    tm.mark(1);
-    make_w<R, NMAX>(n, k + 1, tau, vnext, sh);
-    tm.mark(2);
+    tau = tn;
     cplx<R>* t = vcur;
     vcur = vnext;
     vnext = t;
   }
+  make_w<R, NMAX>(n, n - 2, tau, vcur, sh);  // w of the last reflector
+  tm.mark(1);
   if (tid == 0) {
     cplx<R> a = M.load(M.off(n - 1, n - 1));
     const cplx<R> vv = vcur[n - 1], ww = sh.w[n - 1];

@@ -51,7 +51,7 @@ struct EigParams {
   int ws_ld = 0;
   int* info = nullptr;
   // optional profile (wall clock ticks), EIG_PROF_SLOTS per sample: [0, 8) phase-boundary
-  // timestamps, [8, 16) divide-and-conquer step sums, [16, 20) tridiagonalisation step sums
+  // timestamps, [8, 16) divide-and-conquer step sums, [16, 18) tridiagonalisation step sums
   unsigned long long* prof = nullptr;
 };
 constexpr int EIG_PROF_SLOTS = 24;
