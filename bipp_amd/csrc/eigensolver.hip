@@ -571,6 +571,40 @@ struct CiMfmaTile {
   }
 };
 
+// dc_product_tiles' tile arithmetic in float64 on v_mfma_f64_16x16x4f64 (64 x 64 tiles,
+// 512 threads): 4 x 4 blocks of 16 x 16, two per wavefront (the layouts of gemm_tiles).
+template <int NMAX>
+struct DcMfmaTile {
+  static constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
+  static_assert(TR == 64 && TK % 4 == 0, "64 x 64 tiles");
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  f64x4 acc[2];
+  __device__ void init() {
+    acc[0] = f64x4{0, 0, 0, 0};
+    acc[1] = f64x4{0, 0, 0, 0};
+  }
+  __device__ void step(const double* tA, const double* tB, int tid) {
+    const int lane = tid & 63, wave = tid >> 6, br = wave & 3, bc0 = (wave >> 2) * 2;
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int s4 = 0; s4 < TK / 4; ++s4) {
+      const double a = tA[(4 * s4 + lk) * TR + br * 16 + lr];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tB[(4 * s4 + lk) * TR + (bc0 + q) * 16 + lr],
+                                                      acc[q], 0, 0, 0);
+    }
+  }
+  template <class ST>
+  __device__ void store(ST st, int tid) const {
+    const int lane = tid & 63, wave = tid >> 6, br = wave & 3, bc0 = (wave >> 2) * 2;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) st(br * 16 + (lane >> 4) + 4 * r, (bc0 + q) * 16 + (lane & 15), acc[q][r]);
+  }
+};
+
 // The workgroup as the team of the team-parallel routines (chol_inv.hpp, tridiag_dc.hpp).
 struct WgTeam {
   int tid;
@@ -579,6 +613,10 @@ struct WgTeam {
   template <typename R, int NT>
   using CiTile =
       typename std::conditional<std::is_same<R, double>::value, CiMfmaTile<NT>, CiValuTile<R, NT>>::type;
+  template <typename R, int NMAX, int NT>
+  using DcTile = typename std::conditional<std::is_same<R, double>::value && NT == 512 &&
+                                               DcCfg<NMAX>::TR == 64,
+                                           DcMfmaTile<NMAX>, DcValuTile<R, NMAX, NT>>::type;
   __device__ void sync() const { __syncthreads(); }
   // total over the aligned group of P lanes (a + b == b + a: the same bits in each lane)
   template <int P, typename R>

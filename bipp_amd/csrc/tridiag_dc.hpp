@@ -251,23 +251,54 @@ BIPP_HD int dc_count(const R* d, int st, int b, int e, R x, bool strict) {
   return lo - b;
 }
 
+// Tile arithmetic of dc_product_tiles on the vector units: each thread accumulates an
+// MR x MC block (row group tid mod RG, column group tid / RG).  A team may bring its own
+// (Team::DcTile, the same members), e.g. one on matrix instructions.
+template <typename R, int NMAX, int NT>
+struct DcValuTile {
+  static constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
+  static constexpr int MR = TR * TR / NT < 4 ? TR * TR / NT : 4;
+  static constexpr int MC = TR * TR / (NT * MR);
+  static constexpr int RG = TR / MR;  // row groups (fastest over threads)
+  static_assert(MR >= 1 && MC >= 1 && RG * (TR / MC) == NT, "tile splits over the team");
+  R acc[MR][MC];
+  BIPP_HD void init() {
+    for (int i = 0; i < MR; ++i)
+      for (int j = 0; j < MC; ++j) acc[i][j] = R(0);
+  }
+  // acc += the staged chunk's product (tA, tB: [TK][TR], k-major)
+  BIPP_HD void step(const R* tA, const R* tB, int tid) {
+    const int rg = tid % RG, cg = tid / RG;
+    for (int kk = 0; kk < TK; ++kk) {
+      R a[MR], b[MC];
+      for (int i = 0; i < MR; ++i) a[i] = tA[kk * TR + rg * MR + i];
+      for (int j = 0; j < MC; ++j) b[j] = tB[kk * TR + cg * MC + j];
+      for (int i = 0; i < MR; ++i)
+        for (int j = 0; j < MC; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+  }
+  // st(row, col, value) for the tile's entries (rows and columns relative to the tile)
+  template <class ST>
+  BIPP_HD void store(ST st, int tid) const {
+    const int rg = tid % RG, cg = tid / RG;
+    for (int j = 0; j < MC; ++j)
+      for (int i = 0; i < MR; ++i) st(rg * MR + i, cg * MC + j, acc[i][j]);
+  }
+};
+
 // Q_new = diag(Q1, Q2) M for one merge, in TR x TR tiles (rows of a tile lie in one half:
 // n1 is a multiple of TR), k in chunks of TK staged through LDS, the next chunk's loads in
-// flight while the current one is used.  Each thread accumulates an MR x MC block.
+// flight while the current one is used; the arithmetic is the team's tile type
+// (Team::DcTile).
 template <typename R, int NMAX, int NT, class Team>
 BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBAL R* Bp, int ldp,
                               BIPP_GLOBAL R* Bn, int ldn, const BIPP_GLOBAL R* Mb,
                               const int* slot_of, R* tile) {
   constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
-  constexpr int MR = TR * TR / NT < 4 ? TR * TR / NT : 4;
-  constexpr int MC = TR * TR / (NT * MR);
-  constexpr int RG = TR / MR;                    // row groups (fastest over threads)
   constexpr int LPT = (TK * TR + NT - 1) / NT;   // staged elements per thread and operand
-  static_assert(MR >= 1 && MC >= 1 && RG * (TR / MC) == NT, "tile splits over the team");
   R* tA = tile;            // [TK][TR]: A[ra + r, kc + kk]
   R* tB = tile + TK * TR;  // [TK][TR]: M'[kc + kk, ca + c]
   const int tid = tm.tid;
-  const int rg = tid % RG, cg = tid / RG;
   const int ntr = (sz + TR - 1) / TR;
   for (int t = 0; t < ntr * ntr; ++t) {
     const int ra = (t / ntr) * TR, ca = (t % ntr) * TR;
@@ -285,9 +316,8 @@ BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBA
                     : R(0);
       }
     };
-    R acc[MR][MC];
-    for (int i = 0; i < MR; ++i)
-      for (int j = 0; j < MC; ++j) acc[i][j] = R(0);
+    typename Team::template DcTile<R, NMAX, NT> acc;
+    acc.init();
     load(k0);
     for (int kc = k0; kc < k1; kc += TK) {
       tm.sync();  // the previous chunk is no longer read
@@ -300,21 +330,14 @@ BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBA
       }
       tm.sync();
       if (kc + TK < k1) load(kc + TK);
-      for (int kk = 0; kk < TK; ++kk) {
-        R a[MR], b[MC];
-        for (int i = 0; i < MR; ++i) a[i] = tA[kk * TR + rg * MR + i];
-        for (int j = 0; j < MC; ++j) b[j] = tB[kk * TR + cg * MC + j];
-        for (int i = 0; i < MR; ++i)
-          for (int j = 0; j < MC; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-      }
+      acc.step(tA, tB, tid);
     }
-    for (int j = 0; j < MC; ++j) {
-      const int col = ca + cg * MC + j;
-      for (int i = 0; i < MR; ++i) {
-        const int row = ra + rg * MR + i;
-        if (row < sz && col < sz) Bn[(r0 + row) + (long long)(r0 + col) * ldn] = acc[i][j];
-      }
-    }
+    acc.store(
+        [&](int i, int j, R v) {
+          const int row = ra + i, col = ca + j;
+          if (row < sz && col < sz) Bn[(r0 + row) + (long long)(r0 + col) * ldn] = v;
+        },
+        tid);
   }
   tm.sync();
 }
