@@ -573,10 +573,10 @@ struct CiMfmaTile {
 
 // dc_product_tiles' tile arithmetic in float64 on v_mfma_f64_16x16x4f64 (64 x 64 tiles,
 // 512 threads): 4 x 4 blocks of 16 x 16, two per wavefront (the layouts of gemm_tiles).
-template <int NMAX>
+template <int TK>
 struct DcMfmaTile {
-  static constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
-  static_assert(TR == 64 && TK % 4 == 0, "64 x 64 tiles");
+  static constexpr int TR = 64;
+  static_assert(TK % 4 == 0, "k chunks of whole MFMA steps");
   typedef double f64x4 __attribute__((ext_vector_type(4)));
   f64x4 acc[2];
   __device__ void init() {
@@ -613,10 +613,9 @@ struct WgTeam {
   template <typename R, int NT>
   using CiTile =
       typename std::conditional<std::is_same<R, double>::value, CiMfmaTile<NT>, CiValuTile<R, NT>>::type;
-  template <typename R, int NMAX, int NT>
-  using DcTile = typename std::conditional<std::is_same<R, double>::value && NT == 512 &&
-                                               DcCfg<NMAX>::TR == 64,
-                                           DcMfmaTile<NMAX>, DcValuTile<R, NMAX, NT>>::type;
+  template <typename R, int TR, int TK, int NT>
+  using DcTile = typename std::conditional<std::is_same<R, double>::value && NT == 512 && TR == 64,
+                                           DcMfmaTile<TK>, DcValuTile<R, TR, TK, NT>>::type;
   __device__ void sync() const { __syncthreads(); }
   // total over the aligned group of P lanes (a + b == b + a: the same bits in each lane)
   template <int P, typename R>

@@ -254,9 +254,8 @@ BIPP_HD int dc_count(const R* d, int st, int b, int e, R x, bool strict) {
 // Tile arithmetic of dc_product_tiles on the vector units: each thread accumulates an
 // MR x MC block (row group tid mod RG, column group tid / RG).  A team may bring its own
 // (Team::DcTile, the same members), e.g. one on matrix instructions.
-template <typename R, int NMAX, int NT>
+template <typename R, int TR, int TK, int NT>
 struct DcValuTile {
-  static constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
   static constexpr int MR = TR * TR / NT < 4 ? TR * TR / NT : 4;
   static constexpr int MC = TR * TR / (NT * MR);
   static constexpr int RG = TR / MR;  // row groups (fastest over threads)
@@ -287,14 +286,14 @@ struct DcValuTile {
 };
 
 // Q_new = diag(Q1, Q2) M for one merge, in TR x TR tiles (rows of a tile lie in one half:
-// n1 is a multiple of TR), k in chunks of TK staged through LDS, the next chunk's loads in
-// flight while the current one is used; the arithmetic is the team's tile type
-// (Team::DcTile).
-template <typename R, int NMAX, int NT, class Team>
+// n1 is a multiple of TR), k in chunks of TK staged through LDS (2 TR TK reals of tile), the
+// next chunk's loads in flight while the current one is used; the arithmetic is the team's
+// tile type (Team::DcTile).
+template <typename R, int TR, int NT, class Team>
 BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBAL R* Bp, int ldp,
                               BIPP_GLOBAL R* Bn, int ldn, const BIPP_GLOBAL R* Mb,
                               const int* slot_of, R* tile) {
-  constexpr int TR = DcCfg<NMAX>::TR, TK = DcCfg<NMAX>::TK;
+  constexpr int TK = 16;
   constexpr int LPT = (TK * TR + NT - 1) / NT;   // staged elements per thread and operand
   R* tA = tile;            // [TK][TR]: A[ra + r, kc + kk]
   R* tB = tile + TK * TR;  // [TK][TR]: M'[kc + kk, ca + c]
@@ -316,7 +315,7 @@ BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBA
                     : R(0);
       }
     };
-    typename Team::template DcTile<R, NMAX, NT> acc;
+    typename Team::template DcTile<R, TR, TK, NT> acc;
     acc.init();
     load(k0);
     for (int kc = k0; kc < k1; kc += TK) {
@@ -637,8 +636,16 @@ BIPP_HD void dc_tridiag(Team& tm, int n, R* d, R* e, int st, BIPP_GLOBAL R* Z, i
         const int r0 = m << l;
         if (r0 + h >= n) continue;
         const int sz = n - r0 < S ? n - r0 : S;
-        dc_product_tiles<R, NMAX, NT>(tm, r0, sz, h, Bp, ldp, Bn, ldn, M + (long long)r0 * S,
-                                      sh.slot_of, tile);
+        dc_product_tiles<R, TR, NT>(tm, r0, sz, h, Bp, ldp, Bn, ldn, M + (long long)r0 * S,
+                                    sh.slot_of, tile);
+      }
+    } else if (TR == 64 && h == 32) {
+      for (int m = 0; m < nm; ++m) {
+        const int r0 = m << l;
+        if (r0 + h >= n) continue;
+        const int sz = n - r0 < S ? n - r0 : S;
+        dc_product_tiles<R, 32, NT>(tm, r0, sz, h, Bp, ldp, Bn, ldn, M + (long long)r0 * S,
+                                    sh.slot_of, tile);
       }
     } else {
       // small blocks: one output per thread, consecutive threads down a column

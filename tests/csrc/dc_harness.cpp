@@ -27,8 +27,8 @@ struct CpuTeam {
   double* slot;                                          // one value per thread
   void sync() { bar->arrive_and_wait(); }
   void mark(int) {}
-  template <typename R, int NMAX, int NT>
-  using DcTile = bippamd::DcValuTile<R, NMAX, NT>;
+  template <typename R, int TR, int TK, int NT>
+  using DcTile = bippamd::DcValuTile<R, TR, TK, NT>;
   // the GPU's butterfly over the group: v += v of lane tid ^ m for m = 1, 2, ...
   template <int P, typename R>
   R lane_sum(R v) {
