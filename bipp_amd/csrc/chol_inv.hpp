@@ -12,7 +12,9 @@
 //      ...: inverse of [L11 0; L21 L22] is [W11 0; -W22 L21 W11 W22], with T = L21 W11 in
 //      the scratch array and W21 = -W22 T written over L21.
 // Only the lower triangle of L is read; tiles on the diagonal leave the strict upper
-// triangle untouched.  Team: tid, sync() over NT threads.
+// triangle untouched.  Team: tid, sync() over NT threads, mark(k), which may time step k
+// (0 diagonal factor, 1 diagonal inverse, 2 panel and trailing products, 3 off-diagonal
+// blocks of W).
 #pragma once
 
 #ifndef BIPP_HD
@@ -152,6 +154,7 @@ BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_G
   auto at = [=](int i, int j) -> BIPP_GLOBAL cplx<R>& { return L[i + (long long)j * ld]; };
 
   // ---- 1. blocked right-looking factorisation, diagonal blocks inverted -----------------
+  tm.mark(-1);
   for (int j0 = 0; j0 < n; j0 += CI_CB) {
     const int jb = n - j0 < CI_CB ? n - j0 : CI_CB;
     for (int e = tid; e < CI_CB * CI_CB; e += NT) {
@@ -191,14 +194,27 @@ BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_G
         sh.blk.d[i][k] = i == k ? mk<R>(s, R(0)) : sh.blk.d[i][k] * is;
     }
     tm.sync();
-    // inverse of the factor by forward substitution, one column per thread
-    for (int j = tid; j < jb; j += NT) {
-      sh.blk.w[j][j] = mk<R>(R(1) / sh.blk.d[j][j].re, R(0));
-      for (int i = j + 1; i < jb; ++i) {
-        cplx<R> s = mk<R>(0, 0);
-        for (int k = j; k < i; ++k) s = s + sh.blk.d[i][k] * sh.blk.w[k][j];
-        sh.blk.w[i][j] = s * (R(-1) / sh.blk.d[i][i].re);
+    tm.mark(0);
+    // inverse W = L^-1, right-looking over the whole team: a[i][j] = delta_ij - sum_{m < i}
+    // L_im W_mj with W_mj = a[m][j] / L_mm, accumulated one pivot m per step (the elements
+    // of a step spread over the team); then W_ij = a[i][j] / L_ii
+    for (int e = tid; e < CI_CB * CI_CB; e += NT) {
+      const int i = e % CI_CB, j = e / CI_CB;
+      if (i < jb && i >= j) sh.blk.w[i][j] = mk<R>(i == j ? R(1) : R(0), R(0));
+    }
+    tm.sync();
+    for (int m = 0; m + 1 < jb; ++m) {
+      const R im = R(1) / sh.blk.d[m][m].re;
+      for (int e = tid; e < CI_CB * CI_CB; e += NT) {
+        const int i = e % CI_CB, j = e / CI_CB;
+        if (i > m && i < jb && j <= m)
+          sh.blk.w[i][j] = sh.blk.w[i][j] - sh.blk.d[i][m] * (sh.blk.w[m][j] * im);
       }
+      tm.sync();
+    }
+    for (int e = tid; e < CI_CB * CI_CB; e += NT) {
+      const int i = e % CI_CB, j = e / CI_CB;
+      if (i < jb && i >= j) sh.blk.w[i][j] = sh.blk.w[i][j] * (R(1) / sh.blk.d[i][i].re);
     }
     tm.sync();
     for (int e = tid; e < CI_CB * CI_CB; e += NT) {
@@ -206,6 +222,7 @@ BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_G
       if (i < jb && i >= j) ci_put(at(j0 + i, j0 + j), sh.blk.w[i][j]);
     }
     tm.sync();
+    tm.mark(1);
     const int mp = n - j0 - jb;
     if (mp > 0) {
       const int r0 = j0 + jb;
@@ -236,6 +253,7 @@ BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_G
           },
           sh);
     }
+    tm.mark(2);
   }
 
   // ---- 2. off-diagonal blocks of W, bottom-up ------------------------------------------
@@ -271,6 +289,7 @@ BIPP_HD int chol_inverse(Team& tm, int n, BIPP_GLOBAL cplx<R>* L, int ld, BIPP_G
           sh);
     }
   }
+  tm.mark(3);
   return -1;
 }
 

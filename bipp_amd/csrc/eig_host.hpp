@@ -171,7 +171,8 @@ struct EigRunner {
       BIPP_HIP(hipStreamSynchronize(st));
       static const char* names[7] = {"load", "cholesky", "whiten", "tridiag", "tridiag_eig",
                                      "backtransform", "unwhiten"};
-      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, sub[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tsub[3] = {0, 0, 0};
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0}, sub[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tsub[3] = {0, 0, 0},
+             csub[4] = {0, 0, 0, 0};
       int cnt = 0;
       for (int k = 0; k < nS; ++k) {
         const unsigned long long* r = &h[(size_t)k * EIG_PROF_SLOTS];
@@ -179,7 +180,8 @@ struct EigRunner {
         ++cnt;
         for (int q = 0; q < 7; ++q) acc[q] += r[q + 1] >= r[q] && r[q] ? double(r[q + 1] - r[q]) : 0.0;
         for (int q = 0; q < 8; ++q) sub[q] += double(r[8 + q]);
-        for (int q = 0; q < 3; ++q) tsub[q] += double(r[16 + q]);
+        for (int q = 0; q < 2; ++q) tsub[q] += double(r[16 + q]);
+        for (int q = 0; q < 4; ++q) csub[q] += double(r[18 + q]);
       }
       std::fprintf(stderr, "[bipp eig profile] %d samples, n_slots %d, us/sample (100 MHz clock):", cnt,
                    n_slots);
@@ -193,6 +195,11 @@ struct EigRunner {
       static const char* tsn[2] = {"w, column update and reflector", "pass"};
       std::fprintf(stderr, " | tridiag:");
       for (int q = 0; q < 2; ++q) std::fprintf(stderr, " %s %.1f", tsn[q], cnt ? tsub[q] / cnt / 100.0 : 0.0);
+      // steps of the Cholesky and inverse, summed over its blocks
+      static const char* csn[4] = {"diagonal factor", "diagonal inverse", "panel and trailing",
+                                   "off-diagonal W"};
+      std::fprintf(stderr, " | cholesky:");
+      for (int q = 0; q < 4; ++q) std::fprintf(stderr, " %s %.1f", csn[q], cnt ? csub[q] / cnt / 100.0 : 0.0);
       std::fprintf(stderr, "\n");
     }
 

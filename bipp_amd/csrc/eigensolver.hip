@@ -750,8 +750,9 @@ __device__ __noinline__ void tridiag_phase(int n, cplx<R>* __restrict__ A, int l
 
 // W = L^-1 of the Cholesky factor of G (chol_inv.hpp), out of line; -1 or the failing column.
 template <typename R, int NMAX>
-__device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, int ldt) {
-  WgTeam tm{static_cast<int>(threadIdx.x)};
+__device__ __noinline__ int chol_phase(int n, cplx<R>* L, int ld, cplx<R>* T, int ldt,
+                                       unsigned long long* prof) {
+  WgTeam tm{static_cast<int>(threadIdx.x), uniform(prof)};
   return chol_inverse<R, EigCfg<NMAX>::NT>(tm, uniform(n), global_ptr(L), uniform(ld),
                                            global_ptr(T), uniform(ldt), eig_sh<R, NMAX>.ci);
 }
@@ -1051,7 +1052,8 @@ __device__ void solve_one(const EigParams<R>& p, int sid, EigShared<R, NMAX>& sh
   if (!p.standard) {
     // ---- W = L^-1 of the Cholesky factor G = L L^H (blocked; X is its scratch) ----------
     {
-      const int bad = chol_phase<R, NMAX>(n, L, ld, X, ldx);
+      const int bad = chol_phase<R, NMAX>(n, L, ld, X, ldx,
+                                          p.prof ? p.prof + (size_t)sid * EIG_PROF_SLOTS + 18 : nullptr);
       __syncthreads();
       if (bad >= 0) {
         // leading minor bad + 1 of G is not positive definite: ?hegv reports info = n + bad + 1

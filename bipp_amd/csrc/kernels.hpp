@@ -51,7 +51,8 @@ struct EigParams {
   int ws_ld = 0;
   int* info = nullptr;
   // optional profile (wall clock ticks), EIG_PROF_SLOTS per sample: [0, 8) phase-boundary
-  // timestamps, [8, 16) divide-and-conquer step sums, [16, 18) tridiagonalisation step sums
+  // timestamps, [8, 16) divide-and-conquer step sums, [16, 18) tridiagonalisation step sums,
+  // [18, 22) Cholesky step sums
   unsigned long long* prof = nullptr;
 };
 constexpr int EIG_PROF_SLOTS = 24;

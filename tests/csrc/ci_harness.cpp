@@ -23,6 +23,7 @@ struct CpuTeam {
   int tid;
   std::barrier<>* bar;
   void sync() { bar->arrive_and_wait(); }
+  void mark(int) {}
   template <typename R, int NT>
   using CiTile = bippamd::CiValuTile<R, NT>;
 };
