@@ -164,3 +164,47 @@ def test_batched_ragged_device(ctx):
         assert sc[k] == pytest.approx(sco, rel=1e-15)
         assert np.abs(d[k] - do).max() < 1e-10 * np.abs(do).max()
         assert rel(vdv(v[k], d[k]), vdv(vo, do)) < 1e-10
+
+
+@pytest.mark.parametrize("ragged", [False, True])
+def test_batched_256_bucket(ctx, ragged):
+    """256-antenna pencils (the configuration-4 size: the float64 matrix-instruction phases —
+    Cholesky and whitening tiles, divide-and-conquer products on 64 x 64 and 32 x 32 tiles,
+    the compact-WY back-transform), full or with 180..256 active antennas, in one persistent
+    launch, against the oracle per sample (LAPACK zhegv)."""
+    r = np.random.default_rng(17 + int(ragged))
+    n, ns = 256, 6
+    p0 = synthetic.disk_positions(r, n, 300.0)
+    src, flux = synthetic.zenith_cap_sources(r, 24, 5.0)
+    samples = []
+    for k in range(ns):
+        act = np.ones(n, bool)
+        if ragged:
+            act[:] = False
+            act[r.choice(n, size=int(r.integers(180, n + 1)), replace=False)] = True
+        wl = 2.0 + 0.05 * k
+        s = synthetic.visibilities(p0, wl, src, flux, 0.1)
+        s[~act] = 0
+        s[:, ~act] = 0
+        samples.append(synthetic.Sample(0, k, wl, p0, s, act))
+    s_cm, x_cm, wls = synthetic.stack_samples(samples, np.complex64)
+    s_d = DeviceArray.from_numpy(s_cm)
+    x_d = DeviceArray.from_numpy(x_cm)
+    wl_d = DeviceArray.from_numpy(wls)
+    d_d = DeviceArray.zeros((ns, n), np.float64)
+    v_d = DeviceArray.zeros((ns, n, n), np.complex128)
+    nr_d = DeviceArray.zeros(ns, np.int32)
+    sc_d = DeviceArray.zeros(ns, np.float64)
+    print(f"batched eigensolve, 256 antennas, ragged={ragged}", flush=True)
+    bipp_amd.call("bipp_eigh_gram_batched", ctx.handle, ns, n, n, 0, 1, 1, wl_d.ptr, s_d.ptr, n,
+                  n * n, None, 0, 0, x_d.ptr, n, 3 * n, d_d.ptr, n, v_d.ptr, n, n * n, nr_d.ptr,
+                  sc_d.ptr)
+    ctx.synchronize()
+    d = d_d.numpy()
+    v = v_d.numpy().transpose(0, 2, 1)  # column-major per sample
+    nr = nr_d.numpy()
+    for k, smp in enumerate(samples):
+        vo, do, nro, _ = O.eigh_gram(smp.wl, smp.s.astype(np.complex64), None, smp.xyz)
+        assert nr[k] == nro
+        assert np.abs(d[k] - do).max() < 1e-10 * np.abs(do).max()
+        assert rel(vdv(v[k], d[k]), vdv(vo, do)) < 1e-10
