@@ -44,10 +44,12 @@
 
 namespace bippamd {
 
+constexpr int DC_TK = 16;  // k chunk of the merge products
+
 template <int NMAX>
 struct DcCfg {
   static constexpr int TR = NMAX / 4 < 64 ? NMAX / 4 : 64;  // product tile edge
-  static constexpr int TK = 16;                             // product k chunk
+  static constexpr int TK = DC_TK;                          // product k chunk
   static constexpr int TILE_ELEMS = 2 * TR * TK;            // LDS reals of the tiles
   static constexpr int MAXIT = 100;                         // secular iteration cap
 };
@@ -293,7 +295,7 @@ template <typename R, int TR, int NT, class Team>
 BIPP_HD void dc_product_tiles(Team& tm, int r0, int sz, int n1, const BIPP_GLOBAL R* Bp, int ldp,
                               BIPP_GLOBAL R* Bn, int ldn, const BIPP_GLOBAL R* Mb,
                               const int* slot_of, R* tile) {
-  constexpr int TK = 16;
+  constexpr int TK = DC_TK;
   constexpr int LPT = (TK * TR + NT - 1) / NT;   // staged elements per thread and operand
   R* tA = tile;            // [TK][TR]: A[ra + r, kc + kk]
   R* tB = tile + TK * TR;  // [TK][TR]: M'[kc + kk, ca + c]
