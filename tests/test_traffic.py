@@ -77,3 +77,35 @@ def test_kernel_window_between_copies(tmp_path):
     eig = [l for l in out.splitlines() if "`eig_kernel`" in l][0]
     # the two dispatches between the copies, 0.1 ms each; the one after them is excluded
     assert eig.split("|")[2].strip() == "2" and eig.split("|")[3].strip() == "0.100"
+
+
+def test_counter_window_between_copies(tmp_path):
+    d = tmp_path / "ctr"
+    d.mkdir()
+    # (dispatch, kernel, SQ_WAVES, SQ_INSTS_VALU, start ns, end ns)
+    rows = [(1, "spread_kernel", 10, 1e6, 0, 1000),            # warmup: excluded
+            (2, "copyBuffer", 2048, 5e3, 2000, 3000),          # calibration copy
+            (3, "spread_kernel", 10, 2.4e6, 4000, 1004000),    # timed, 1 ms
+            (4, "copyBuffer", 4, 1e2, 1005000, 1006000),       # a small copy in the window
+            (5, "copyBuffer", 2048, 5e3, 1007000, 1008000),    # calibration copy
+            (6, "spread_kernel", 10, 9e9, 1009000, 1010000)]   # after: excluded
+    fields = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value", "VGPR_Count",
+              "Start_Timestamp", "End_Timestamp"]
+    with open(d / "run_counter_collection.csv", "w", newline="") as fh:
+        w = csv.DictWriter(fh, fields)
+        w.writeheader()
+        for did, n, waves, valu, t0, t1 in rows:
+            for c, v in (("SQ_WAVES", waves), ("SQ_INSTS_VALU", valu)):
+                for part in (0.5, 0.5):   # two instances per dispatch, summed by the reader
+                    w.writerow({"Dispatch_Id": did, "Kernel_Name": n, "Counter_Name": c,
+                                "Counter_Value": v * part, "VGPR_Count": 32,
+                                "Start_Timestamp": t0, "End_Timestamp": t1})
+    out = subprocess.run([sys.executable, "-m", "bipp_amd.kernel_window", "--counters", str(d)],
+                         check=True, cwd=ROOT, capture_output=True, text=True).stdout
+    body = [l for l in out.splitlines() if l.startswith("| `")]
+    assert len(body) == 1 and "`spread_kernel`" in body[0]
+    cells = [c.strip() for c in body[0].split("|")[1:-1]]
+    assert cells[1] == "1.000" and cells[2] == "32"
+    assert cells[3] == "2.4e+06" and cells[4] == "10"
+    # 4 cycles per wave64 instruction over 1024 SIMDs x 2.4e9 Hz x 1 ms
+    assert cells[5] == f"{4 * 2.4e6 / (1024 * 2.4e9 * 1e-3):.2f}"
