@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace bippamd {
 
@@ -1159,7 +1160,8 @@ __global__ void pos_range_kernel(const double* pos, long long pos_stride, int ap
   }
 }
 
-// VV_l = Z diag(w_l) Z^H for every (sample, level); Z rows = antennas used (ld ldz).
+// VV_l = Z diag(w_l) Z^H for every (sample, level); Z rows = antennas used (ld ldz).  The
+// vector-ALU form, used for float32 eigenvectors (float64 ones take vv_levels_mfma_kernel).
 template <typename R, typename TO>
 __global__ __launch_bounds__(256) void vv_levels_kernel(N3VVParams<R> p) {
   // 64 x 64 output tile per workgroup (tiles on or below the diagonal), eigenvectors taken
@@ -1218,6 +1220,107 @@ __global__ __launch_bounds__(256) void vv_levels_kernel(N3VVParams<R> p) {
       const int i = i0 + tx + 16 * u, j = j0 + ty + 16 * v;
       if (i < n && j < n && i >= j) out[i + (size_t)p.ld_vv * j] = cvt<TO>(acc[u][v]);
     }
+}
+
+// vv_levels_kernel in float64 on v_mfma_f64_16x16x4f64 with the operands loaded straight from
+// global memory (no LDS, no barriers).  Workgroup = (sample, level, 64 x 64 tile on or below
+// the diagonal); wavefront = 32 x 32 quadrant as 2 x 2 blocks of 16 x 16.  The products form
+// D[j][i] = sum_e conj(Z[j, e]) (w_e Z[i, e]) = VV[i][j] (A operand: lane l holds row j =
+// l % 16, eigenvector l / 16; B operand: eigenvector l / 16, row i = l % 16; result register r:
+// j = l / 16 + 4 r, i = l % 16), so one store writes 16 consecutive rows i of a column j.  Only
+// the eigenvectors from the level's first to its last nonzero weight are visited, four per
+// step, with the next step's operands loaded before the current step's products.  Workgroups
+// are numbered so that the tiles and levels of one sample run on one XCD (workgroup x runs on
+// XCD x mod 8), which keeps the sample's Z in that XCD's L2.  Held to 128 registers (four
+// wavefronts per SIMD; the 2 x 2 blocks' sums stay in vector registers, no spills).
+template <typename TO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
+vv_levels_mfma_kernel(N3VVParams<double> p, int n_tiles, int per_xcd, int n_units) {
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  const int unit = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (unit >= n_units) return;
+  const int t = unit % n_tiles, sl = unit / n_tiles;
+  const int s = sl / p.n_levels, lv = sl % p.n_levels;
+  int ti = 0;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int n = p.n_rows ? p.n_rows[s] : p.n_rows_const;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ib = 64 * ti + 32 * (wave & 1), jb = 64 * tj + 32 * (wave >> 1);
+  // quadrant outside the matrix or strictly above the diagonal (every i < every j)
+  if (ib >= n || jb >= n || jb > ib + 31) return;
+  const double* dw = p.wd + ((size_t)s * p.n_levels + lv) * p.n_cols;
+  int lo = p.n_cols, hi = -1;
+  for (int e = lane; e < p.n_cols; e += 64)
+    if (dw[e] != 0.0) {
+      lo = e < lo ? e : lo;
+      hi = e;
+    }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const int ol = __shfl_xor(lo, m), oh = __shfl_xor(hi, m);
+    lo = ol < lo ? ol : lo;
+    hi = oh > hi ? oh : hi;
+  }
+  const cplx<double>* Z = p.z + (size_t)s * p.stride_z;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int jr[2] = {jb + lr, jb + 16 + lr}, ir[2] = {ib + lr, ib + 16 + lr};
+  f64x4 dr[2][2], di[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      dr[a][b] = f64x4{0, 0, 0, 0};
+      di[a][b] = f64x4{0, 0, 0, 0};
+    }
+  // P = Z[j, e] (rows j of the two column blocks), Q = w_e Z[i, e] (rows i of the two row
+  // blocks) for the eigenvector e = k + l / 16 of this lane; zero outside [lo, hi] and n.
+  auto load = [&](int k, cplx<double>* P, cplx<double>* Q) {
+    const int e = k + lk;
+    const bool ok = e <= hi;
+    const double w = ok ? dw[e] : 0.0;
+    const cplx<double>* col = Z + (size_t)(ok ? e : 0) * p.ldz;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      P[b] = (ok && jr[b] < n) ? col[jr[b]] : mk<double>(0, 0);
+      const cplx<double> z = (ok && ir[b] < n) ? col[ir[b]] : mk<double>(0, 0);
+      Q[b] = mk<double>(z.re * w, z.im * w);
+    }
+  };
+  if (lo <= hi) {
+    cplx<double> P[2], Q[2];
+    load(lo & ~3, P, Q);
+    for (int k = lo & ~3; k <= hi; k += 4) {
+      cplx<double> Pn[2], Qn[2];
+      load(k + 4, Pn, Qn);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          // re: Pr Qr + Pi Qi; im: Pr Qi - Pi Qr
+          dr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[a].re, Q[b].re, dr[a][b], 0, 0, 0);
+          dr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[a].im, Q[b].im, dr[a][b], 0, 0, 0);
+          di[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[a].re, Q[b].im, di[a][b], 0, 0, 0);
+          di[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-P[a].im, Q[b].re, di[a][b], 0, 0, 0);
+        }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        P[b] = Pn[b];
+        Q[b] = Qn[b];
+      }
+    }
+  }
+  cplx<TO>* out = reinterpret_cast<cplx<TO>*>(p.vv) + (size_t)sl * p.vv_stride;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = jb + 16 * a + lk + 4 * r, i = ib + 16 * b + lr;
+        if (i < n && j < n && i >= j)
+          out[i + (size_t)p.ld_vv * j] = mk<TO>(static_cast<TO>(dr[a][b][r]), static_cast<TO>(di[a][b][r]));
+      }
 }
 
 // Trace of VV_l for every (sample, level) into the unused upper-triangle slot (0, 1).
@@ -1540,9 +1643,19 @@ template void launch_vv_trace<double>(int, int, const int*, int, void*, long lon
 template <typename R, typename TO>
 void launch_vv_levels(const N3VVParams<R>& p, int n_samples, int n_rows_max, hipStream_t st) {
   if (n_samples <= 0 || n_rows_max <= 0) return;
-  dim3 grid(ceil_div(n_rows_max, 64), ceil_div(n_rows_max, 64), n_samples * p.n_levels);
-  vv_levels_kernel<R, TO><<<grid, 256, 0, st>>>(p);
-  check_launch("vv_levels_kernel");
+  if constexpr (std::is_same<R, double>::value) {
+    const long long nt = ceil_div(n_rows_max, 64), n_tiles = nt * (nt + 1) / 2;
+    const long long n_units = n_tiles * n_samples * p.n_levels;
+    const long long per_xcd = (n_units + 7) / 8;
+    if (8 * per_xcd > 0x7fffffffLL) fail(BIPP_INVALID_PARAMETER_ERROR, "too many VV tiles for one launch");
+    vv_levels_mfma_kernel<TO><<<static_cast<unsigned>(8 * per_xcd), 256, 0, st>>>(
+        p, static_cast<int>(n_tiles), static_cast<int>(per_xcd), static_cast<int>(n_units));
+    check_launch("vv_levels_mfma_kernel");
+  } else {
+    dim3 grid(ceil_div(n_rows_max, 64), ceil_div(n_rows_max, 64), n_samples * p.n_levels);
+    vv_levels_kernel<R, TO><<<grid, 256, 0, st>>>(p);
+    check_launch("vv_levels_kernel");
+  }
 }
 template void launch_vv_levels<double, float>(const N3VVParams<double>&, int, int, hipStream_t);
 template void launch_vv_levels<double, double>(const N3VVParams<double>&, int, int, hipStream_t);
