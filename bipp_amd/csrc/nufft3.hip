@@ -687,24 +687,24 @@ __device__ __forceinline__ void dft_small(cplx<R>* v) {
     dft_ct<R, 4, 4>(v);
   } else {
     static_assert(RAD == 5, "radix 2, 3, 4, 5, 6, 8, 9 or 16");
-    // radix 5: roots exp(+2 pi i k / 5)
+    // radix 5, roots w^k = exp(+2 pi i k / 5) = c_k + i s_k with w^4 = conj(w), w^3 =
+    // conj(w^2): with t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3,
+    //   y1, y4 = x0 + c1 t1 + c2 t2 +- i (s1 t3 + s2 t4)
+    //   y2, y3 = x0 + c2 t1 + c1 t2 +- i (s2 t3 - s1 t4)
+    // (36 real operations instead of the 80 of the direct sums)
     const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
     const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
-    const R cr[5] = {R(1), c1, c2, c2, c1};
-    const R sr[5] = {R(0), s1, s2, -s2, -s1};
-    cplx<R> o[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      cplx<R> acc = v[0];
-#pragma unroll
-      for (int r = 1; r < 5; ++r) {
-        const int e = (r * k) % 5;
-        acc = acc + v[r] * mk<R>(cr[e], sr[e]);
-      }
-      o[k] = acc;
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) v[k] = o[k];
+    const cplx<R> x0 = v[0];
+    const cplx<R> t1 = v[1] + v[4], t2 = v[2] + v[3], t3 = v[1] - v[4], t4 = v[2] - v[3];
+    const cplx<R> a1 = mk<R>(x0.re + c1 * t1.re + c2 * t2.re, x0.im + c1 * t1.im + c2 * t2.im);
+    const cplx<R> a2 = mk<R>(x0.re + c2 * t1.re + c1 * t2.re, x0.im + c2 * t1.im + c1 * t2.im);
+    const cplx<R> b1 = mk<R>(s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im);
+    const cplx<R> b2 = mk<R>(s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im);
+    v[0] = x0 + t1 + t2;
+    v[1] = mk<R>(a1.re - b1.im, a1.im + b1.re);
+    v[4] = mk<R>(a1.re + b1.im, a1.im - b1.re);
+    v[2] = mk<R>(a2.re - b2.im, a2.im + b2.re);
+    v[3] = mk<R>(a2.re + b2.im, a2.im - b2.re);
   }
 }
 
@@ -733,17 +733,27 @@ __device__ __forceinline__ int fast_div(int x, unsigned m, int s) {
 }
 
 // One Stockham stage of radix RAD over nl lines (stride ld) of length n; (nb_m, nb_s) and
-// (ns_m, ns_s) divide by n / RAD and by ns.
-template <typename R, int RAD>
+// (ns_m, ns_s) divide by n / RAD and by ns.  FIRST: the first stage (ns = 1), whose
+// twiddles are all 1 (k = 0), so none are formed or applied.
+template <typename R, int RAD, bool FIRST>
 __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n, int nl, int ld,
                                           int ns, const Twid<R>& tw, int tid, int nt,
                                           unsigned nb_m, int nb_s, unsigned ns_m, int ns_s) {
   const int nb = n / RAD;
-  const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
   for (int t = tid; t < nl * nb; t += nt) {
     const int line = fast_div(t, nb_m, nb_s), j = t - line * nb;
     const cplx<R>* src = in + (size_t)line * ld;
     cplx<R>* dst = out + (size_t)line * ld;
+    if constexpr (FIRST) {
+      cplx<R> v[RAD];
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[r] = src[j + r * nb];
+      dft_small<R, RAD>(v);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) dst[j * RAD + r] = v[r];
+      continue;
+    }
+    const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
     const int jq = fast_div(j, ns_m, ns_s);
     const int k = j - jq * ns;
     // twiddles w^r, w = exp(2 pi i k tstride / n): w, w^4 and w^8 from the table, the
@@ -778,6 +788,32 @@ __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n
 // compiled in (5: radices 2-5 only, which keeps the kernel's register and scalar-register
 // footprint small; 16: also the composite radices 6, 8, 9, 16); the launchers pick it from
 // the plan.
+template <typename R, int MAXR, bool FIRST>
+__device__ __forceinline__ void fft_stage_radix(int R_, const cplx<R>* in, cplx<R>* out, int n,
+                                                int nl, int ld, int ns, const N3Radix& rad, int st,
+                                                const Twid<R>& tw, int tid, int nt) {
+#define DIVS rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st]
+  switch (R_) {
+    case 2: fft_stage<R, 2, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+    case 3: fft_stage<R, 3, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+    case 4: fft_stage<R, 4, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+    case 6:
+      if constexpr (MAXR >= 6) fft_stage<R, 6, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
+      break;
+    case 8:
+      if constexpr (MAXR >= 8) fft_stage<R, 8, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
+      break;
+    case 9:
+      if constexpr (MAXR >= 9) fft_stage<R, 9, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
+      break;
+    case 16:
+      if constexpr (MAXR >= 16) fft_stage<R, 16, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
+      break;
+    default: fft_stage<R, 5, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
+  }
+#undef DIVS
+}
+
 template <typename R, int MAXR>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
                             const Twid<R>& tw, int tid, int nt) {
@@ -786,26 +822,10 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
   cplx<R>* out = b;
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
-#define DIVS rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st]
-    switch (R_) {
-      case 2: fft_stage<R, 2>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-      case 3: fft_stage<R, 3>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-      case 4: fft_stage<R, 4>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-      case 6:
-        if constexpr (MAXR >= 6) fft_stage<R, 6>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-        break;
-      case 8:
-        if constexpr (MAXR >= 8) fft_stage<R, 8>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-        break;
-      case 9:
-        if constexpr (MAXR >= 9) fft_stage<R, 9>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-        break;
-      case 16:
-        if constexpr (MAXR >= 16) fft_stage<R, 16>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-        break;
-      default: fft_stage<R, 5>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-    }
-#undef DIVS
+    if (st == 0)
+      fft_stage_radix<R, MAXR, true>(R_, in, out, n, nl, ld, ns, rad, st, tw, tid, nt);
+    else
+      fft_stage_radix<R, MAXR, false>(R_, in, out, n, nl, ld, ns, rad, st, tw, tid, nt);
     __syncthreads();
     ns *= R_;
     cplx<R>* tmp = in;
