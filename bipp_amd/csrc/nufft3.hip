@@ -734,98 +734,133 @@ __device__ __forceinline__ int fast_div(int x, unsigned m, int s) {
 
 // One Stockham stage of radix RAD over nl lines (stride ld) of length n; (nb_m, nb_s) and
 // (ns_m, ns_s) divide by n / RAD and by ns.  FIRST: the first stage (ns = 1), whose
-// twiddles are all 1 (k = 0), so none are formed or applied.
-template <typename R, int RAD, bool FIRST>
+// twiddles are all 1 (k = 0), so none are formed or applied.  TAB: the twiddles w^(k r) of
+// this stage are read from its table (entry (r - 1) ns + k, build_stage_tab) instead of
+// being formed from the two-level table.
+template <typename R, int RAD, bool FIRST, bool TAB>
 __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n, int nl, int ld,
-                                          int ns, const Twid<R>& tw, int tid, int nt,
-                                          unsigned nb_m, int nb_s, unsigned ns_m, int ns_s) {
+                                          int ns, const Twid<R>& tw, const cplx<R>* stab,
+                                          int tid, int nt, unsigned nb_m, int nb_s,
+                                          unsigned ns_m, int ns_s) {
   const int nb = n / RAD;
   for (int t = tid; t < nl * nb; t += nt) {
     const int line = fast_div(t, nb_m, nb_s), j = t - line * nb;
-    const cplx<R>* src = in + (size_t)line * ld;
+    const cplx<R>* src = in + (size_t)line * ld + j;
     cplx<R>* dst = out + (size_t)line * ld;
+    cplx<R> v[RAD];
     if constexpr (FIRST) {
-      cplx<R> v[RAD];
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) v[r] = src[j + r * nb];
+      for (int r = 0; r < RAD; ++r) v[r] = src[r * nb];
       dft_small<R, RAD>(v);
+      dst += j * RAD;
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) dst[j * RAD + r] = v[r];
+      for (int r = 0; r < RAD; ++r) dst[r] = v[r];
       continue;
     }
-    const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
     const int jq = fast_div(j, ns_m, ns_s);
     const int k = j - jq * ns;
-    // twiddles w^r, w = exp(2 pi i k tstride / n): w, w^4 and w^8 from the table, the
-    // others as products at most two deep (k r tstride < n for r < RAD)
-    const int t1 = k * tstride;
-    cplx<R> wp[RAD];
-    if constexpr (RAD > 1) wp[1] = tw(t1);
-    if constexpr (RAD > 2) wp[2] = wp[1] * wp[1];
-    if constexpr (RAD > 3) wp[3] = wp[2] * wp[1];
-    if constexpr (RAD > 4) {
-      wp[4] = tw(4 * t1);
+    v[0] = src[0];
+    if constexpr (TAB) {
+      const cplx<R>* tk = stab + k;  // consecutive lanes read consecutive entries
 #pragma unroll
-      for (int r = 5; r < RAD && r < 8; ++r) wp[r] = wp[4] * wp[r - 4];
+      for (int r = 1; r < RAD; ++r) v[r] = src[r * nb] * tk[(r - 1) * ns];
+    } else {
+      // twiddles w^r, w = exp(2 pi i k tstride / n): w, w^4 and w^8 from the table, the
+      // others as products at most two deep (k r tstride < n for r < RAD)
+      const int tstride = n / (ns * RAD);  // twiddle k r / (ns RAD) of the full length
+      const int t1 = k * tstride;
+      cplx<R> wp[RAD];
+      if constexpr (RAD > 1) wp[1] = tw(t1);
+      if constexpr (RAD > 2) wp[2] = wp[1] * wp[1];
+      if constexpr (RAD > 3) wp[3] = wp[2] * wp[1];
+      if constexpr (RAD > 4) {
+        wp[4] = tw(4 * t1);
+#pragma unroll
+        for (int r = 5; r < RAD && r < 8; ++r) wp[r] = wp[4] * wp[r - 4];
+      }
+      if constexpr (RAD > 8) {
+        wp[8] = tw(8 * t1);
+#pragma unroll
+        for (int r = 9; r < RAD; ++r) wp[r] = wp[8] * wp[r - 8];
+      }
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) v[r] = src[r * nb] * wp[r];
     }
-    if constexpr (RAD > 8) {
-      wp[8] = tw(8 * t1);
-#pragma unroll
-      for (int r = 9; r < RAD; ++r) wp[r] = wp[8] * wp[r - 8];
-    }
-    cplx<R> v[RAD];
-    v[0] = src[j];
-#pragma unroll
-    for (int r = 1; r < RAD; ++r) v[r] = src[j + r * nb] * wp[r];
     dft_small<R, RAD>(v);
-    const int idx = jq * ns * RAD + k;
+    dst += jq * ns * RAD + k;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) dst[idx + r * ns] = v[r];
+    for (int r = 0; r < RAD; ++r) dst[r * ns] = v[r];
   }
+}
+
+// Twiddle tables of the stages after the first, one after another: stage st (span ns, radix
+// RAD) holds RAD - 1 rows of ns entries, entry (r - 1) ns + k = exp(2 pi i k r tstride / n)
+// with tstride = n / (ns RAD) (k r tstride < n); n - r[0] entries in all.
+__host__ __device__ __forceinline__ int stage_tab_len(int n, const N3Radix& rad) {
+  return rad.count > 1 ? n - rad.r[0] : 0;
+}
+template <typename R>
+__device__ void build_stage_tab(cplx<R>* tab, int n, const N3Radix& rad, const Twid<R>& tw,
+                                int tid, int nt) {
+  int ns = rad.r[0], off = 0;
+  for (int st = 1; st < rad.count; ++st) {
+    const int RAD = rad.r[st], rm = RAD - 1, cnt = ns * rm;
+    const int tstride = n / (ns * RAD);
+    for (int e = tid; e < cnt; e += nt) {
+      const int r = e / ns + 1, k = e - (r - 1) * ns;
+      tab[off + e] = tw(k * r * tstride);
+    }
+    off += cnt;
+    ns *= RAD;
+  }
+}
+
+template <typename R, int MAXR, bool FIRST, bool TAB>
+__device__ __forceinline__ void fft_stage_radix(int R_, const cplx<R>* in, cplx<R>* out, int n,
+                                                int nl, int ld, int ns, const N3Radix& rad, int st,
+                                                const Twid<R>& tw, const cplx<R>* stab, int tid,
+                                                int nt) {
+#define STAGE(RR) fft_stage<R, RR, FIRST, TAB>(in, out, n, nl, ld, ns, tw, stab, tid, nt, \
+                                               rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st])
+  switch (R_) {
+    case 2: STAGE(2); break;
+    case 3: STAGE(3); break;
+    case 4: STAGE(4); break;
+    case 6:
+      if constexpr (MAXR >= 6) STAGE(6);
+      break;
+    case 8:
+      if constexpr (MAXR >= 8) STAGE(8);
+      break;
+    case 9:
+      if constexpr (MAXR >= 9) STAGE(9);
+      break;
+    case 16:
+      if constexpr (MAXR >= 16) STAGE(16);
+      break;
+    default: STAGE(5); break;
+  }
+#undef STAGE
 }
 
 // Stockham stages in LDS; returns the buffer holding the result.  MAXR: the largest radix
 // compiled in (5: radices 2-5 only, which keeps the kernel's register and scalar-register
 // footprint small; 16: also the composite radices 6, 8, 9, 16); the launchers pick it from
 // the plan.
-template <typename R, int MAXR, bool FIRST>
-__device__ __forceinline__ void fft_stage_radix(int R_, const cplx<R>* in, cplx<R>* out, int n,
-                                                int nl, int ld, int ns, const N3Radix& rad, int st,
-                                                const Twid<R>& tw, int tid, int nt) {
-#define DIVS rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st]
-  switch (R_) {
-    case 2: fft_stage<R, 2, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-    case 3: fft_stage<R, 3, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-    case 4: fft_stage<R, 4, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-    case 6:
-      if constexpr (MAXR >= 6) fft_stage<R, 6, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-      break;
-    case 8:
-      if constexpr (MAXR >= 8) fft_stage<R, 8, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-      break;
-    case 9:
-      if constexpr (MAXR >= 9) fft_stage<R, 9, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-      break;
-    case 16:
-      if constexpr (MAXR >= 16) fft_stage<R, 16, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS);
-      break;
-    default: fft_stage<R, 5, FIRST>(in, out, n, nl, ld, ns, tw, tid, nt, DIVS); break;
-  }
-#undef DIVS
-}
-
-template <typename R, int MAXR>
+template <typename R, int MAXR, bool TAB>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
-                            const Twid<R>& tw, int tid, int nt) {
+                            const Twid<R>& tw, const cplx<R>* stab, int tid, int nt) {
   int ns = 1;
   cplx<R>* in = a;
   cplx<R>* out = b;
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
-    if (st == 0)
-      fft_stage_radix<R, MAXR, true>(R_, in, out, n, nl, ld, ns, rad, st, tw, tid, nt);
-    else
-      fft_stage_radix<R, MAXR, false>(R_, in, out, n, nl, ld, ns, rad, st, tw, tid, nt);
+    if (st == 0) {
+      fft_stage_radix<R, MAXR, true, false>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
+    } else {
+      fft_stage_radix<R, MAXR, false, TAB>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
+      if (TAB) stab += ns * (R_ - 1);
+    }
     __syncthreads();
     ns *= R_;
     cplx<R>* tmp = in;
@@ -848,7 +883,8 @@ constexpr int FFT_PF = 8;
 
 // Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).  The
 // next group's inputs are fetched into registers while the current group is transformed.
-template <typename R, int MAXR>
+// TAB: the stages' twiddle tables are built in LDS once per workgroup (build_stage_tab).
+template <typename R, int MAXR, bool TAB>
 __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, int nl) {
   extern __shared__ __align__(16) unsigned char fsh[];
   __shared__ long long lbase[2][16];
@@ -857,6 +893,11 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
   cplx<R>* a = reinterpret_cast<cplx<R>*>(fsh);
   cplx<R>* b = a + (size_t)nl * n;
   const Twid<R> tw = load_twid<R>(p.tw, n, b + (size_t)nl * n, tid, FFT_NT);
+  cplx<R>* stab = b + (size_t)nl * n + twid_len(n);
+  if (TAB) {
+    __syncthreads();  // the two-level table is in LDS
+    build_stage_tab<R>(stab, n, p.rad, tw, tid, FFT_NT);
+  }
   const long long n_lines = (long long)p.n_levels * p.zw.count * p.yw.count;
   const long long n_groups = (n_lines + nl - 1) / nl;
   auto set_bases = [&](long long grp, int buf) {
@@ -903,7 +944,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
     if (gn < n_groups) set_bases(gn, cb ^ 1);
     __syncthreads();
     if (gn < n_groups) fetch(gn, cb ^ 1, nxt);  // in flight during the transform
-    cplx<R>* res = lds_fft<R, MAXR>(a, b, n, nlg, n, p.rad, tw, tid, FFT_NT);
+    cplx<R>* res = lds_fft<R, MAXR, TAB>(a, b, n, nlg, n, p.rad, tw, stab, tid, FFT_NT);
     for (int i = tid; i < nlg * n; i += FFT_NT) {
       const int li = fast_div(i, p.rad.n_m, p.rad.n_s), k = i - li * n;
       if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
@@ -963,7 +1004,8 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
       a[xi * ld + i] = v;
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R, MAXR>(a, b, p.m, TW, ld, p.rad, tw, threadIdx.x, blockDim.x);
+    cplx<R>* res = lds_fft<R, MAXR, false>(a, b, p.m, TW, ld, p.rad, tw, nullptr, threadIdx.x,
+                                           blockDim.x);
     for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
       const int xi = t & (TW - 1), k = t >> tw_shift;
       const int j = (xt << tw_shift) + xi;
@@ -1489,9 +1531,13 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
   // several short lines per workgroup so that every stage keeps most lanes busy
   const int nl = std::max(1, std::min(16, 2048 / p.n));
   if ((long long)nl * p.n > FFT_NT * FFT_PF) fail(BIPP_INTERNAL_ERROR, "FFT line group too long");
-  const size_t sh = sizeof(cplx<R>) * (2 * (size_t)p.n * nl + twid_len(p.n));
+  const size_t sh0 = sizeof(cplx<R>) * (2 * (size_t)p.n * nl + twid_len(p.n));
   // 1 KiB of the 160 KiB stays for the kernel's static line bases
-  if (sh > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
+  if (sh0 > 159 * 1024) fail(BIPP_INTERNAL_ERROR, "FFT line exceeds LDS");
+  // the stages' twiddle tables where two workgroups still fit a CU
+  const size_t sh_tab = sh0 + sizeof(cplx<R>) * stage_tab_len(p.n, p.rad);
+  const bool tab = sh_tab <= 79 * 1024;
+  const size_t sh = tab ? sh_tab : sh0;
   const long long groups = (n_lines + nl - 1) / nl;
   const unsigned blocks = static_cast<unsigned>(std::min<long long>(groups, 32768));
   auto go = [&](auto kernel, bool& attr_set) {
@@ -1502,11 +1548,18 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
     }
     kernel<<<blocks, FFT_NT, sh, st>>>(p, nl);
   };
-  static bool attr5 = false, attr16 = false;
-  if (max_radix(p.rad) > 5)
-    go(&fft_lines_kernel<R, 16>, attr16);
-  else
-    go(&fft_lines_kernel<R, 5>, attr5);
+  static bool attr5 = false, attr16 = false, attr5t = false, attr16t = false;
+  if (max_radix(p.rad) > 5) {
+    if (tab)
+      go(&fft_lines_kernel<R, 16, true>, attr16t);
+    else
+      go(&fft_lines_kernel<R, 16, false>, attr16);
+  } else {
+    if (tab)
+      go(&fft_lines_kernel<R, 5, true>, attr5t);
+    else
+      go(&fft_lines_kernel<R, 5, false>, attr5);
+  }
   check_launch("fft_lines_kernel");
 }
 template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
