@@ -736,17 +736,17 @@ __device__ __forceinline__ int fast_div(int x, unsigned m, int s) {
 // (ns_m, ns_s) divide by n / RAD and by ns.  FIRST: the first stage (ns = 1), whose
 // twiddles are all 1 (k = 0), so none are formed or applied.  TAB: the twiddles w^(k r) of
 // this stage are read from its table (entry (r - 1) ns + k, build_stage_tab) instead of
-// being formed from the two-level table.
-template <typename R, int RAD, bool FIRST, bool TAB>
+// being formed from the two-level table.  ONE: a single line (nl = 1), no line index.
+template <typename R, int RAD, bool FIRST, bool TAB, bool ONE>
 __device__ __forceinline__ void fft_stage(const cplx<R>* in, cplx<R>* out, int n, int nl, int ld,
                                           int ns, const Twid<R>& tw, const cplx<R>* stab,
                                           int tid, int nt, unsigned nb_m, int nb_s,
                                           unsigned ns_m, int ns_s) {
   const int nb = n / RAD;
-  for (int t = tid; t < nl * nb; t += nt) {
-    const int line = fast_div(t, nb_m, nb_s), j = t - line * nb;
-    const cplx<R>* src = in + (size_t)line * ld + j;
-    cplx<R>* dst = out + (size_t)line * ld;
+  for (int t = tid; t < (ONE ? nb : nl * nb); t += nt) {
+    const int line = ONE ? 0 : fast_div(t, nb_m, nb_s), j = ONE ? t : t - line * nb;
+    const cplx<R>* src = ONE ? in + j : in + (size_t)line * ld + j;
+    cplx<R>* dst = ONE ? out : out + (size_t)line * ld;
     cplx<R> v[RAD];
     if constexpr (FIRST) {
 #pragma unroll
@@ -815,12 +815,12 @@ __device__ void build_stage_tab(cplx<R>* tab, int n, const N3Radix& rad, const T
   }
 }
 
-template <typename R, int MAXR, bool FIRST, bool TAB>
+template <typename R, int MAXR, bool FIRST, bool TAB, bool ONE>
 __device__ __forceinline__ void fft_stage_radix(int R_, const cplx<R>* in, cplx<R>* out, int n,
                                                 int nl, int ld, int ns, const N3Radix& rad, int st,
                                                 const Twid<R>& tw, const cplx<R>* stab, int tid,
                                                 int nt) {
-#define STAGE(RR) fft_stage<R, RR, FIRST, TAB>(in, out, n, nl, ld, ns, tw, stab, tid, nt, \
+#define STAGE(RR) fft_stage<R, RR, FIRST, TAB, ONE>(in, out, n, nl, ld, ns, tw, stab, tid, nt, \
                                                rad.nb_m[st], rad.nb_s[st], rad.ns_m[st], rad.ns_s[st])
   switch (R_) {
     case 2: STAGE(2); break;
@@ -847,7 +847,7 @@ __device__ __forceinline__ void fft_stage_radix(int R_, const cplx<R>* in, cplx<
 // compiled in (5: radices 2-5 only, which keeps the kernel's register and scalar-register
 // footprint small; 16: also the composite radices 6, 8, 9, 16); the launchers pick it from
 // the plan.
-template <typename R, int MAXR, bool TAB>
+template <typename R, int MAXR, bool TAB, bool ONE>
 __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const N3Radix& rad,
                             const Twid<R>& tw, const cplx<R>* stab, int tid, int nt) {
   int ns = 1;
@@ -856,9 +856,9 @@ __device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int n, int nl, int ld, const
   for (int st = 0; st < rad.count; ++st) {
     const int R_ = rad.r[st];
     if (st == 0) {
-      fft_stage_radix<R, MAXR, true, false>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
+      fft_stage_radix<R, MAXR, true, false, ONE>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
     } else {
-      fft_stage_radix<R, MAXR, false, TAB>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
+      fft_stage_radix<R, MAXR, false, TAB, ONE>(R_, in, out, n, nl, ld, ns, rad, st, tw, stab, tid, nt);
       if (TAB) stab += ns * (R_ - 1);
     }
     __syncthreads();
@@ -884,7 +884,9 @@ constexpr int FFT_PF = 8;
 // Contiguous axis (x): `nl` lines per workgroup (lines (level, z, y) of the windows).  The
 // next group's inputs are fetched into registers while the current group is transformed.
 // TAB: the stages' twiddle tables are built in LDS once per workgroup (build_stage_tab).
-template <typename R, int MAXR, bool TAB>
+// ONE: one line per group (nl = 1): no line index per element, and the stores run over the
+// output window only.
+template <typename R, int MAXR, bool TAB, bool ONE>
 __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, int nl) {
   extern __shared__ __align__(16) unsigned char fsh[];
   __shared__ long long lbase[2][16];
@@ -917,6 +919,15 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
   };
   auto fetch = [&](long long grp, int buf, cplx<R>* regs) {
     const int nlg = static_cast<int>(min<long long>(nl, n_lines - grp * nl));
+    if (ONE) {
+      const cplx<R>* g = p.grid + lbase[buf][0];
+#pragma unroll
+      for (int k = 0; k < FFT_PF; ++k) {
+        const int i = tid + k * FFT_NT;
+        if (i < n) regs[k] = in_window(i, p.in, n) ? g[i] : mk<R>(0, 0);
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < FFT_PF; ++k) {
       const int i = tid + k * FFT_NT;
@@ -944,10 +955,19 @@ __global__ __launch_bounds__(FFT_NT) void fft_lines_kernel(N3LineParams<R> p, in
     if (gn < n_groups) set_bases(gn, cb ^ 1);
     __syncthreads();
     if (gn < n_groups) fetch(gn, cb ^ 1, nxt);  // in flight during the transform
-    cplx<R>* res = lds_fft<R, MAXR, TAB>(a, b, n, nlg, n, p.rad, tw, stab, tid, FFT_NT);
-    for (int i = tid; i < nlg * n; i += FFT_NT) {
-      const int li = fast_div(i, p.rad.n_m, p.rad.n_s), k = i - li * n;
-      if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
+    cplx<R>* res = lds_fft<R, MAXR, TAB, ONE>(a, b, n, nlg, n, p.rad, tw, stab, tid, FFT_NT);
+    if (ONE) {
+      cplx<R>* g = p.grid + lbase[cb][0];
+      for (int e = tid; e < p.out.count; e += FFT_NT) {
+        int k = p.out.start + e;  // start < n, count <= n
+        k -= k >= n ? n : 0;
+        g[k] = res[k];
+      }
+    } else {
+      for (int i = tid; i < nlg * n; i += FFT_NT) {
+        const int li = fast_div(i, p.rad.n_m, p.rad.n_s), k = i - li * n;
+        if (in_window(k, p.out, n)) p.grid[lbase[cb][li] + k] = res[i];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -1004,7 +1024,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(N3PassParams<R> p, int
       a[xi * ld + i] = v;
     }
     __syncthreads();
-    cplx<R>* res = lds_fft<R, MAXR, false>(a, b, p.m, TW, ld, p.rad, tw, nullptr, threadIdx.x,
+    cplx<R>* res = lds_fft<R, MAXR, false, false>(a, b, p.m, TW, ld, p.rad, tw, nullptr, threadIdx.x,
                                            blockDim.x);
     for (int t = threadIdx.x; t < (p.m << tw_shift); t += blockDim.x) {
       const int xi = t & (TW - 1), k = t >> tw_shift;
@@ -1548,18 +1568,17 @@ void launch_fft_lines(const N3LineParams<R>& p, hipStream_t st) {
     }
     kernel<<<blocks, FFT_NT, sh, st>>>(p, nl);
   };
-  static bool attr5 = false, attr16 = false, attr5t = false, attr16t = false;
-  if (max_radix(p.rad) > 5) {
-    if (tab)
-      go(&fft_lines_kernel<R, 16, true>, attr16t);
-    else
-      go(&fft_lines_kernel<R, 16, false>, attr16);
-  } else {
-    if (tab)
-      go(&fft_lines_kernel<R, 5, true>, attr5t);
-    else
-      go(&fft_lines_kernel<R, 5, false>, attr5);
-  }
+  // [max radix > 5][stage tables][one line per group]
+  static bool attr[2][2][2] = {};
+  const bool r16 = max_radix(p.rad) > 5, one = nl == 1;
+  if (r16 && tab && one) go(&fft_lines_kernel<R, 16, true, true>, attr[1][1][1]);
+  else if (r16 && tab) go(&fft_lines_kernel<R, 16, true, false>, attr[1][1][0]);
+  else if (r16 && one) go(&fft_lines_kernel<R, 16, false, true>, attr[1][0][1]);
+  else if (r16) go(&fft_lines_kernel<R, 16, false, false>, attr[1][0][0]);
+  else if (tab && one) go(&fft_lines_kernel<R, 5, true, true>, attr[0][1][1]);
+  else if (tab) go(&fft_lines_kernel<R, 5, true, false>, attr[0][1][0]);
+  else if (one) go(&fft_lines_kernel<R, 5, false, true>, attr[0][0][1]);
+  else go(&fft_lines_kernel<R, 5, false, false>, attr[0][0][0]);
   check_launch("fft_lines_kernel");
 }
 template void launch_fft_lines<float>(const N3LineParams<float>&, hipStream_t);
