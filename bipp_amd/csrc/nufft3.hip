@@ -128,8 +128,9 @@ __device__ __forceinline__ bool slot_point(const N3Source& p, int s, long long t
 }
 
 // Point records: one thread per listed slot (or per slot of every sample); counts per bin
-// (fill = 0), or the records and values written into their bins' sorted ranges (fill = 1).
-// Both launches compute a point's bin from the same operations.
+// and each point's rank in its bin (fill = 0), or the records and values written into their
+// bins' sorted ranges at those ranks (fill = 1).  Both launches compute a point's bin from
+// the same operations.
 __global__ void n3_points_kernel(N3PointsParams p) {
   const long long sps = p.src.slots_per_sample();
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -157,10 +158,10 @@ __global__ void n3_points_kernel(N3PointsParams p) {
   }
   const int binid = (bin[2] * p.nbins[1] + bin[1]) * p.nbins[0] + bin[0];
   if (!p.fill) {
-    atomicAdd(&p.bin_count[binid], 1u);
+    p.rank[i] = atomicAdd(&p.bin_count[binid], 1u);
     return;
   }
-  const unsigned dst = p.bin_start[binid] + atomicAdd(&p.bin_fill[binid], 1u);
+  const unsigned dst = p.bin_start[binid] + p.rank[i];
   double ps, pc;
   if (p.phase_fp32) {
     float fs, fc;

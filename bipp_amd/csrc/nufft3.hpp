@@ -85,14 +85,15 @@ struct N3PointsParams {
   const void* vv;                // [nS][n_levels][vv_stride] complex (vv_double: double)
   long long vv_stride;
   int n_levels, vv_double;
-  // Two launches over the same points: the first (fill = 0) counts the points per bin; the
-  // second (fill = 1), after the scan of the counts into bin_start, writes each point's
-  // record and values straight into its bin's range of the sorted arrays (slot claimed with
-  // bin_fill), so the unsorted records are never stored.
+  // Two launches over the same points: the first (fill = 0) counts the points per bin and
+  // keeps each point's arrival rank in its bin; the second (fill = 1), after the scan of the
+  // counts into bin_start, writes each point's record and values straight into its slot
+  // bin_start + rank of the sorted arrays (no atomics), so the unsorted records are never
+  // stored.
   int fill;
-  unsigned* bin_count;           // [n_bins] (fill = 0)
+  unsigned* bin_count;           // [n_bins] zeroed (fill = 0)
+  unsigned* rank;                // [n_list] written (fill = 0), read (fill = 1)
   const unsigned* bin_start;     // [n_bins] (fill = 1)
-  unsigned* bin_fill;            // [n_bins] zeroed (fill = 1)
   N3Point* pts;                  // [points] sorted by bin (fill = 1)
   void* vals;                    // [points][n_levels] complex, same order (fill = 1)
 };

@@ -519,7 +519,7 @@ inline void report_spread_profile(const unsigned long long* dprof, long long nb,
 template <typename TG>
 struct N3Runner {
   DevBuf grid, sorted, sorted_vals;
-  DevBuf bin_count, bin_start, bin_fill, scan_tmp, vv, deconv, gl, twid, prof_buf;
+  DevBuf bin_count, bin_start, bin_rank, scan_tmp, vv, deconv, gl, twid, prof_buf;
   DevBuf group_of, group_count, group_start, group_fill, group_tmp, slot_list, ranges;
   std::vector<cplx<TG>> htw;
   std::vector<TG> hdec;
@@ -770,19 +770,17 @@ struct N3Runner {
     // counting sort without an unsorted copy: count per bin, scan, then every point written
     // into its bin's range
     pp.bin_count = bin_count.as<unsigned>(nb);
+    pp.rank = bin_rank.as<unsigned>(n_list);
     launch_zero<unsigned>(pp.bin_count, nb, st);
     pp.fill = 0;
     launch_n3_points(pp, nS, st);
     unsigned* bstart = bin_start.as<unsigned>(nb);
     unsigned* btmp = scan_tmp.as<unsigned>(3 * ((size_t)nb / 1024 + 2) + 4096);
     launch_scan(pp.bin_count, bstart, btmp, (int)nb, st);
-    unsigned* bfill = bin_fill.as<unsigned>(nb);
-    launch_zero<unsigned>(bfill, nb, st);
     N3Point* srt = sorted.as<N3Point>(n_list);
     cplx<TG>* svals = sorted_vals.as<cplx<TG>>((size_t)n_list * L);
     pp.fill = 1;
     pp.bin_start = bstart;
-    pp.bin_fill = bfill;
     pp.pts = srt;
     pp.vals = svals;
     launch_n3_points(pp, nS, st);
