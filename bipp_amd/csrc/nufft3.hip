@@ -176,12 +176,9 @@ __global__ void n3_points_kernel(N3PointsParams p) {
     const double f = turns - rint(turns);
     sincospi(2.0 * f, &ps, &pc);
   }
-  pt.ph[0] = static_cast<float>(pc);
-  pt.ph[1] = static_cast<float>(ps);
-  pt.src = sp.src;
-  pt.mult = sp.mult;
-  pt.sample = s;
-  pt.flags = sp.conj;
+  // the multiplicity (1 or 2) scales the pre-phase exactly
+  pt.ph[0] = static_cast<float>(pc) * sp.mult;
+  pt.ph[1] = static_cast<float>(ps) * sp.mult;
   p.pts[dst] = pt;
   // the point's values of every level, conjugated for an upper-triangle entry
   const float sgn = sp.conj ? -1.f : 1.f;
@@ -475,15 +472,14 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
       st_row8(sb.pwx[slot], row[0]);
       st_row8(sb.pwy[slot], row[1]);
       st_row8(sb.pwz[slot], row[2]);
-      const R m = R(pt.mult);
-      const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);
+      const R pr = R(pt.ph[0]), pim = R(pt.ph[1]);  // multiplicity included
       R pvr[SP_LG * 2];
 #pragma unroll
       for (int lv = 0; lv < SP_LG; ++lv) {
         if (lv < L) {
           const TV c = vals[lv];
-          pvr[2 * lv] = m * (R(c.re) * pr - R(c.im) * pim);
-          pvr[2 * lv + 1] = m * (R(c.re) * pim + R(c.im) * pr);
+          pvr[2 * lv] = R(c.re) * pr - R(c.im) * pim;
+          pvr[2 * lv + 1] = R(c.re) * pim + R(c.im) * pr;
         } else {
           pvr[2 * lv] = pvr[2 * lv + 1] = R(0);
         }

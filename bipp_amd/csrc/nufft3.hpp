@@ -26,16 +26,14 @@ struct N3Radix {
   int nb_s[20] = {}, ns_s[20] = {};
 };
 
-// One non-uniform point of the spread.
+// One non-uniform point of the spread (32 bytes; its level values are stored beside it).
 struct alignas(16) N3Point {
   int l0[3];    // first cell of the kernel support, spread index + nf1/2
-  int src;      // offset of the value VV(a, b) inside the sample's matrix (lower triangle)
   float fr[3];  // cell - (l0 - nf1/2) per axis, in [w/2, w/2 + 1)
-  float mult;   // 2 for a pair whose mirror point is its conjugate, else 1
-  float ph[2];  // pre-phase exp(i u'.D)
-  int sample;
-  int flags;    // bit 0: the value is conj(VV(src)) (an upper-triangle entry)
+  float ph[2];  // multiplicity times the pre-phase exp(i u'.D) (2 for a pair whose mirror
+                // point is its conjugate, else 1)
 };
+static_assert(sizeof(N3Point) == 32, "point records are 32 bytes");
 
 // Where the input points of a batch come from.
 //   N3_SRC_PAIRS: antenna pairs a >= b of each sample, u = 2 pi (p_a - p_b) from positions
