@@ -329,8 +329,9 @@ __global__ void scan_add_kernel(unsigned* out, const unsigned* block_offsets, in
 // sums are reduced through LDS and the block is written once, times the deconvolution
 // factors, with plain stores: no atomics on the grid, and every cell of [-nf1/2, nf1/2)^3
 // written by exactly one workgroup (no zero pass over the grid).  The sums follow the order
-// of the points inside their bins, which the sort fills in arrival order (slots claimed with
-// an atomic counter): float images agree between runs to rounding, not bit for bit.
+// of the points inside their bins, which the counting pass ranks in arrival order (the
+// values an atomic counter returns): float images agree between runs to rounding, not bit
+// for bit.
 // --------------------------------------------------------------------------------------
 template <typename R>
 struct SpreadBatch {
