@@ -462,14 +462,16 @@ __global__ __launch_bounds__(256) void n3_spread_kernel(N3SpreadParams<R> p) {
     if (hit) {
       R row[3][B];
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+      for (int d = 0; d < 3; ++d) {
+        // column c is support offset o = c - rel of the point, at (o - fr) 2 / w: the
+        // argument stepped from c = 0, the support test 0 <= o < w as one unsigned compare
+        const R x0 = (R(-rel[d]) - R(pt.fr[d])) * two_over_w;
 #pragma unroll
-        for (int c = 0; c < B; ++c) {
-          const int o = c - rel[d];
-          row[d][c] = (o >= 0 && o < w)
-                          ? es_kernel<R>((R(o) - R(pt.fr[d])) * two_over_w, beta)
+        for (int c = 0; c < B; ++c)
+          row[d][c] = static_cast<unsigned>(c - rel[d]) < static_cast<unsigned>(w)
+                          ? es_kernel<R>(x0 + R(c) * two_over_w, beta)
                           : R(0);
-        }
+      }
       st_row8(sb.pwx[slot], row[0]);
       st_row8(sb.pwy[slot], row[1]);
       st_row8(sb.pwz[slot], row[2]);
